@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark of the FPS hot path on one MI355X.
+
+A step is one vanilla FPS run (fps_sampling semantics, src/lib.cpp:188-246)
+over one synthetic cloud already resident in HBM: one call of
+fpsx_fps_sampling_device, i.e. the exchange-area clear and the single
+persistent FPS launch that makes all M-1 passes.
+
+Default workload (the metric's configuration): one cloud of N = 1,048,576
+points, D = 3 float32, a mixture of 64 isotropic Gaussian clusters (sigma
+0.02, centres uniform in [0,1)^3, numpy default_rng(1)), M = 16,384 samples,
+start_idx = 0.
+
+Prints one JSON line.  value = samples/s of the whole job (M x steps / wall
+time between two device synchronisations).  roofline: algorithmic bytes of
+the vanilla loop, 20 B per point-pass (x, y, z and min-distance read, the
+min-distance written) x N x (M - 1) per launch, divided by the FPS kernel's
+average duration measured with HIP events on the stream it runs on, against
+the 8 TB/s HBM peak.  cpu_baseline: the reference's fps_sampling compiled
+from its own sources (oracle/_ref) -- or the oracle's C restatement where
+that build is absent -- timed single-threaded on this host over a bounded
+number of samples of the same cloud (its per-sample cost does not depend on
+how many samples are drawn).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_POINT_PASS = 20
+
+
+# ---------------------------------------------------------------- workloads
+def gen_uniform_4096(seed: int = 42) -> np.ndarray:
+    """N=4096 uniform in [0,1)^3 from np.random.seed(42) (bench/test_bench.py:19-21)."""
+    np.random.seed(seed)
+    return np.random.rand(4096, 3).astype(np.float32)
+
+
+def gen_gauss_mixture(n: int = 1 << 20, seed: int = 1) -> np.ndarray:
+    """64 isotropic Gaussian clusters, sigma 0.02, centres uniform in [0,1)^3."""
+    rng = np.random.default_rng(seed)
+    centers = rng.random((64, 3))
+    lab = rng.integers(0, 64, n)
+    return (centers[lab] + 0.02 * rng.standard_normal((n, 3))).astype(np.float32)
+
+
+def gen_lattice_dups(n: int = 1 << 24, seed: int = 4) -> np.ndarray:
+    """Integer lattice {0..255}^3: 75% distinct sites, 25% exact copies, shuffled."""
+    rng = np.random.default_rng(seed)
+    distinct = n * 3 // 4
+    sites = rng.choice(1 << 24, distinct, replace=False)
+    copies = sites[rng.integers(0, distinct, n - distinct)]
+    codes = np.concatenate([sites, copies])
+    rng.shuffle(codes)
+    return np.stack([(codes >> 16) & 255, (codes >> 8) & 255, codes & 255],
+                    axis=1).astype(np.float32)
+
+
+WORKLOADS = {
+    "gauss1m": dict(gen=gen_gauss_mixture, m=16384,
+                    desc="one cloud N=1048576 D=3 f32, 64 Gaussian clusters sigma 0.02, "
+                         "M=16384, start_idx=0"),
+    "uniform4k": dict(gen=gen_uniform_4096, m=1024,
+                      desc="one cloud N=4096 D=3 f32 uniform [0,1)^3 seed 42, M=1024, start_idx=0"),
+    "lattice16m": dict(gen=gen_lattice_dups, m=4096,
+                       desc="one cloud N=16777216 D=3 f32 lattice {0..255}^3, 25% duplicates, "
+                            "M=4096, start_idx=0"),
+}
+
+
+# ---------------------------------------------------------------- CPU baseline
+def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float) -> dict:
+    from oracle import oracle as O
+
+    ref = O.load_reference()
+    if ref is not None:
+        kind = "reference"
+        run = lambda m: ref.fps_sampling(pc, m, 0)  # noqa: E731
+    else:
+        kind = "port"
+        run = lambda m: O.fps(pc, m, 0)  # noqa: E731
+    n = pc.shape[0]
+    # calibrate on a few passes, then size the sample to about target_s seconds
+    m_cal = max(2, min(m_full, int(2e8 // max(n, 1)) + 2))
+    t0 = time.perf_counter()
+    run(m_cal)
+    dt = time.perf_counter() - t0
+    per_pass = dt / max(m_cal - 1, 1)
+    m_s = int(min(m_full, max(m_cal, target_s / max(per_pass, 1e-12))))
+    t0 = time.perf_counter()
+    run(m_s)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(m_s / dt, 3),
+        "unit": "samples/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"first {m_s} of the {m_full} samples of the same cloud, start_idx=0, "
+                  f"{dt:.2f} s single-threaded on {platform.processor() or platform.machine()} "
+                  f"(os.cpu_count()={os.cpu_count()})",
+    }
+
+
+def _read_traffic(workload: str):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"fps_kernel_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch="):
+            try:
+                return float(line.split("=", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="gauss1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: FPS of one cloud runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
+        return 2
+
+    from fpsample_amd import device as D
+
+    wl = WORKLOADS[args.workload]
+    m = wl["m"]
+    pc = wl["gen"]()
+    n = pc.shape[0]
+    soa = np.ascontiguousarray(pc.T)  # (3, N): coordinate j of point i at j*N + i
+    d_pts = D.DeviceArray.from_numpy(soa)
+    d_start = D.DeviceArray.from_numpy(np.zeros(1, np.uint64))
+    d_out = D.DeviceArray((m,), np.uint64)
+    plan = D.plan_info(n, 3)
+
+    def step():
+        D.fps_sampling_device(d_pts, n, 3, m, d_start, 1, d_out)
+
+    for _ in range(args.warmup):
+        step()
+    D.synchronize()
+    D.device_status()
+
+    D.timing(True)
+    D.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    D.synchronize()
+    t1 = time.perf_counter()
+    kernel_ms, launches = D.timing_read()
+    D.timing(False)
+    D.device_status()
+
+    out = d_out.numpy()
+    assert out[0] == 0 and len(np.unique(out)) == m, "picks failed the sanity check"
+
+    wall = t1 - t0
+    ms_per_step = wall / args.steps * 1e3
+    value = m * args.steps / wall
+    kernel_s = kernel_ms / max(launches, 1) / 1e3
+    alg_bytes = BYTES_PER_POINT_PASS * n * (m - 1)
+    achieved = alg_bytes / kernel_s / 1e9
+    result = {
+        "metric": "samples/s at N=1M,M=16K (+% roofline)" if args.workload == "gauss1m"
+        else f"samples/s ({args.workload})",
+        "value": round(value, 3),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "n_points": n, "n_samples": m, "dim": 3,
+                   "plan": plan},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": _read_traffic(args.workload),
+            "kernel_ms": round(kernel_s * 1e3, 4),
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+        "cpu_baseline": None,
+    }
+    if args.cpu_baseline == "on":
+        result["cpu_baseline"] = cpu_baseline(pc, m, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,128 @@
+"""MI355X farthest point sampling -- drop-in for the reference's numpy API.
+
+    import fpsample_amd as fpsample
+    idx = fpsample.fps_sampling(pc, 1024, start_idx=0)
+    idx = fpsample.bucket_fps_kdline_sampling(pc, 1024, h=7, start_idx=0)
+
+The functions keep the reference's names, signatures, defaults, assertions,
+dtype coercion and the single draw from numpy's global RNG when start_idx is
+None (src/fpsample/__init__.py:19-65, 174-206).  The work runs on the GPU
+through the pybind11 module ``fpsample_amd._fpsample`` and the C ABI of
+``libfpsx.so``; there is no CPU fallback: importing this package fails when
+the native extension is missing.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Union
+
+import numpy as np
+
+try:
+    from ._fpsample import (  # noqa: F401
+        __doc__ as _ext_doc,
+        __version__,
+        _bucket_fps_kdline_sampling,
+        _fps_sampling,
+    )
+except ImportError as exc:  # pragma: no cover - exercised only on a broken install
+    raise ImportError(
+        "fpsample_amd: the native extension (_fpsample + libfpsx.so) is not built; "
+        "run `python -m fpsample_amd.build`"
+    ) from exc
+
+
+def get_start_idx(n_pts: int, start_idx: Optional[Union[int, List[int]]]) -> Union[int, np.ndarray]:
+    """Resolve start_idx as src/fpsample/__init__.py:19-32 does.
+
+    None draws one index from numpy's global RNG (np.random.randint), an int is
+    used as is, a list becomes a uint64 array, anything else raises ValueError.
+    """
+    if start_idx is None:
+        start_idx = np.random.randint(low=0, high=n_pts)
+    elif isinstance(start_idx, int):
+        start_idx = start_idx
+    elif isinstance(start_idx, list):
+        start_idx = np.array(start_idx, dtype=np.uint64)
+    else:
+        raise ValueError("start_idx should be None, int or list")
+    return start_idx
+
+
+def fps_sampling(
+    pc: np.ndarray, n_samples: int, start_idx: Optional[Union[int, List[int]]] = None
+) -> np.ndarray:
+    """
+    Vanilla FPS sampling (src/fpsample/__init__.py:35-65), run on the GPU.
+
+    Args:
+        pc (np.ndarray): The input point cloud of shape (n_pts, D).
+        n_samples (int): Number of samples.
+        start_idx (int or list[int], default=None): The starting index of sampling. If set to None, it will be randomly picked.
+            If multiple start indices are given, they are used as a start set for the FPS and will all be present in the final samples.
+    Returns:
+        np.ndarray: The selected indices of shape (n_samples,), dtype uint64.
+    """
+    assert n_samples >= 1, "n_samples should be >= 1"
+    assert pc.ndim == 2
+    n_pts, _ = pc.shape
+    assert n_pts >= n_samples, "n_pts should be >= n_samples"
+    if isinstance(start_idx, int):
+        assert (
+            start_idx is None or 0 <= start_idx < n_pts
+        ), "start_idx should be None or 0 <= start_idx < n_pts"
+    if isinstance(start_idx, list):
+        assert len(start_idx) <= n_samples, "len(start_idx) should be <= n_samples"
+        for idx in start_idx:
+            assert 0 <= idx < n_pts, "start_idx should be None or 0 <= start_idx < n_pts"
+    # Fortran order float32 is the SoA layout the GPU kernels read.
+    pc = np.asfortranarray(pc, dtype=np.float32)
+    start_idx = get_start_idx(n_pts, start_idx)
+    return _fps_sampling(pc, n_samples, start_idx)
+
+
+def bucket_fps_kdline_sampling(
+    pc: np.ndarray, n_samples: int, h: int, start_idx: Optional[Union[int, List[int]]] = None
+) -> np.ndarray:
+    """
+    Bucket-based FPS using a KD-line tree ("QuickFPS"), src/fpsample/__init__.py:174-206.
+
+    Returns exactly the reference's picks: the tree of height ``h`` fixes the
+    point order, the first pick is the point at position ``start_idx`` of that
+    order, and ties go to the lowest position.  The sampling itself runs on the
+    GPU as exhaustive FPS over the tree order, which selects the same points as
+    the reference's pruned bucket search.
+
+    Args:
+        pc (np.ndarray): The input point cloud of shape (n_pts, D), 1 <= D <= 8.
+        n_samples (int): Number of samples.
+        h (int): Height of the KD tree (2**h buckets).
+        start_idx (int, default=None): Position of the first pick in the tree order; random if None.
+
+    Returns:
+        np.ndarray: The selected indices of shape (n_samples,), dtype uint64.
+    """
+    assert n_samples >= 1, "n_samples should be >= 1"
+    assert pc.ndim == 2
+    n_pts, _ = pc.shape
+    assert n_pts >= n_samples, "n_pts should be >= n_samples"
+    assert h >= 1, "h should be >= 1"
+    assert 2**h <= n_pts, "2**h should be <= n_pts"
+    assert (
+        start_idx is None or 0 <= start_idx < n_pts
+    ), "start_idx should be None or 0 <= start_idx < n_pts"
+    if isinstance(start_idx, list):
+        assert len(start_idx) <= n_samples, "len(start_idx) should be <= n_samples"
+    pc = np.ascontiguousarray(pc, dtype=np.float32)
+    start_idx = get_start_idx(n_pts, start_idx)
+    return _bucket_fps_kdline_sampling(pc, n_samples, h, start_idx)
+
+
+__doc__ = _ext_doc
+
+__all__ = [
+    "__doc__",
+    "__version__",
+    "fps_sampling",
+    "bucket_fps_kdline_sampling",
+    "get_start_idx",
+]

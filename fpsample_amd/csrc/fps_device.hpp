@@ -1,0 +1,141 @@
+// fps_device.hpp -- device-side building blocks of the FPS kernels (gfx950).
+//
+// Values: a min-distance is a float32 in [+0, +inf] (never NaN, never -0), so
+// its bit pattern read as int32 orders exactly like the float.  Padding lanes
+// carry -1.0f, whose int32 pattern is negative and loses to every real value.
+//
+// Exchange area (device memory, uint64 words):  word(buf, w, g) at
+//   xchg[(buf * kSlotWords + w) * G + g]
+// holds word w of workgroup g's candidate for exchange buffer buf (0/1):
+//   w = 0 value bits, w = 1 point index, w = 2 + j coordinate j bits.
+// Each word is (payload << 32) | tag with tag = exchange number (1, 2, ...).
+// Every 64-bit word carries its own tag, so a reader never needs ordering
+// between words: it polls until every word it needs shows the tag it expects.
+// Exchange e uses buffer e & 1.  A workgroup writes exchange e only after it
+// has read every candidate of exchange e - 1, which every workgroup publishes
+// only after finishing its reads of exchange e - 2: the buffer being
+// overwritten is never still being read.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fpsx {
+
+constexpr int kTieHighest = 0;
+constexpr int kTieLowest = 1;
+constexpr int kSlotWords = 16;  // value, index, up to 14 coordinates
+constexpr int kMaxXchgDim = kSlotWords - 2;
+
+// Status bits written by kernels (device word, cleared before each launch).
+constexpr uint32_t kStatusTimeout = 1u;
+
+// ---------------------------------------------------------------------------
+// DPP wave reductions (64-lane wavefront).  Result is read from lane 63,
+// so every lane returns the same (uniform) value.
+//   quad_perm[1,0,3,2], quad_perm[2,3,0,1]  -> max within each quad
+//   row_shr:4, row_shr:8                    -> lane 15 of a row holds the row max
+//   row_bcast:15 (rows 1,3), row_bcast:31 (rows 2,3) -> lane 63 holds the wave max
+// Lanes a DPP move cannot source from keep their own value (old = src), which
+// max tolerates because it is idempotent.
+// ---------------------------------------------------------------------------
+#define FPSX_DPP(v, ctrl, rmask) __builtin_amdgcn_update_dpp((v), (v), (ctrl), (rmask), 0xf, false)
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+  v = max(v, FPSX_DPP(v, 0xb1, 0xf));
+  v = max(v, FPSX_DPP(v, 0x4e, 0xf));
+  v = max(v, FPSX_DPP(v, 0x114, 0xf));
+  v = max(v, FPSX_DPP(v, 0x118, 0xf));
+  v = max(v, FPSX_DPP(v, 0x142, 0xa));
+  v = max(v, FPSX_DPP(v, 0x143, 0xc));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t u) {
+  int v = (int)u;
+  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0xb1, 0xf));
+  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x4e, 0xf));
+  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x114, 0xf));
+  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x118, 0xf));
+  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x142, 0xa));
+  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x143, 0xc));
+  return (uint32_t)__builtin_amdgcn_readlane(v, 63);
+}
+
+#undef FPSX_DPP
+
+__device__ __forceinline__ int readlane_i(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// True when candidate (v, i) beats (bv, bi) under the tie rule.
+template <int TIE>
+__device__ __forceinline__ bool beats(int v, uint32_t i, int bv, uint32_t bi) {
+  if (TIE == kTieHighest)
+    return v > bv || (v == bv && i > bi);
+  return v > bv || (v == bv && i < bi);
+}
+
+// Lane holding the wave's best (value, index) under the tie rule; the wave's
+// best value is returned through wv.  Lanes without a candidate pass INT_MIN.
+// The common case -- one lane holds the maximum -- costs one value reduction
+// and a ballot; only real ties pay for an index reduction.
+template <int TIE>
+__device__ __forceinline__ int wave_argmax_lane(int v, uint32_t idx, int &wv) {
+  wv = wave_max_i32(v);
+  const uint64_t tied = __ballot(v == wv);
+  if (__popcll(tied) == 1)
+    return __ffsll((unsigned long long)tied) - 1;
+  const uint32_t key = (v == wv) ? (TIE == kTieHighest ? idx : ~idx) : 0u;
+  const uint32_t km = wave_max_u32(key);
+  const uint32_t widx = (TIE == kTieHighest) ? km : ~km;
+  return __ffsll((unsigned long long)__ballot(v == wv && idx == widx)) - 1;
+}
+
+// Exact `d < m ? d : m` for d in [+0, +inf] or NaN and m never NaN.
+// v_min_f32 returns the non-NaN operand for a quiet NaN (IEEE mode), and a
+// NaN distance produced by arithmetic is always quiet, so one instruction
+// reproduces the reference's `if (dist < dist_min[i]) dist_min[i] = dist`.
+__device__ __forceinline__ float min_keep(float d, float m) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(d), "v"(m));
+  return r;
+}
+
+// Squared distance in the reference's operand order: ((d0*d0 + d1*d1) + d2*d2)...,
+// every product and sum separately rounded (compiled with -ffp-contract=off).
+template <int D>
+__device__ __forceinline__ float sqdist(const float (&p)[D], const float (&q)[D]) {
+  float df = p[0] - q[0];
+  float acc = df * df;
+#pragma unroll
+  for (int j = 1; j < D; ++j) {
+    float dj = p[j] - q[j];
+    acc = acc + dj * dj;
+  }
+  return acc;
+}
+
+// Select arr[k] for a wave-uniform k.
+template <int PPT>
+__device__ __forceinline__ float pick_uniform(const float (&arr)[PPT], int k) {
+  float r = arr[0];
+#pragma unroll
+  for (int q = 1; q < PPT; ++q)
+    r = (k == q) ? arr[q] : r;
+  return r;
+}
+
+// Agent-scope (cross-XCD coherent) relaxed 64-bit word access.
+__device__ __forceinline__ void xchg_store(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t xchg_load(const uint64_t *p) {
+  return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
+
+}  // namespace fpsx

@@ -1,0 +1,46 @@
+// fps_kernels.hpp -- host-visible description of the FPS kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fpsx {
+
+// Status bits a kernel may set in FpsParams::status.
+constexpr uint32_t kStatusBadStart = 2u;  // a start index >= n (replaced by 0)
+
+// Kernel arguments (passed by value).
+struct FpsParams {
+  const float *pts;       // SoA: coordinate j of point i at pts[j * ld + i]
+  uint64_t ld;            // >= n
+  uint32_t n;             // number of points (< 2^31)
+  uint32_t m;             // number of samples (1..n)
+  const uint64_t *starts; // n_starts forced picks (device)
+  uint32_t n_starts;      // >= 1
+  uint32_t dim;           // runtime dimension (generic streaming kernel)
+  uint64_t *out;          // m indices (device)
+  uint64_t *xchg;         // exchange area, 2 * kSlotWords * G words, zeroed before launch
+  uint32_t *status;       // status word, zeroed before launch
+  float *dmin;            // streaming kernels: n min-distances, filled with init
+  uint32_t chunk;         // streaming kernels: points per workgroup
+  float init;             // initial min-distance
+  uint64_t spin_ticks;    // realtime ticks an exchange may wait before giving up
+};
+
+struct KernelEntry {
+  int dim;    // 0 = any dimension
+  int block;  // threads per workgroup
+  int ppt;    // points per thread (resident kernels), 0 for streaming
+  int tie;    // 0 highest, 1 lowest
+  const void *fn;
+};
+
+const KernelEntry *resident_table(size_t *count);
+const KernelEntry *stream_table(size_t *count);
+
+void launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t st);
+void launch_fill_f32(float *p, size_t n, float v, hipStream_t st);
+void launch_rows_to_soa(const float *rows, float *soa, uint32_t n, uint32_t dim, uint64_t ld,
+                        hipStream_t st);
+
+}  // namespace fpsx

@@ -1,0 +1,471 @@
+// fpsx_api.hip -- host runtime and C ABI of libfpsx.so (declared in include/fpsx.h).
+//
+// Per device the library keeps one context: a non-blocking stream at the
+// default priority, grow-only device buffers (points, starts, output,
+// exchange area, status word, streamed min-distances) and the occupancy of
+// every kernel.  A call plans a launch (kernel shape and grid, see plan()),
+// uploads, clears the exchange area and status word with fill kernels,
+// launches once, and copies the picks back.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fps_device.hpp"
+#include "fps_kernels.hpp"
+#include "fpsx.h"
+#include "fpsx_internal.hpp"
+#include "kdline.hpp"
+
+#define FPSX_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace fpsx {
+
+static thread_local std::string t_err;
+
+int fail(int code, const std::string &msg) {
+  t_err = msg;
+  return code;
+}
+
+static int hip_fail(hipError_t e, const char *what) {
+  return fail(e == hipErrorOutOfMemory ? FPSX_ENOMEM : FPSX_EHIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                  \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return hip_fail(_e, #expr);  \
+  } while (0)
+
+hipError_t DevBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return hipSuccess;
+  if (p) {
+    hipError_t e = hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (e != hipSuccess) return e;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  hipError_t e = hipMalloc(&p, want);
+  if (e == hipSuccess) cap = want;
+  return e;
+}
+
+DevBuf::~DevBuf() {
+  // Device memory is released with the process; contexts live until exit.
+}
+
+static std::mutex g_ctx_mu;
+static std::map<int, std::unique_ptr<Ctx>> g_ctx;
+
+int get_ctx(Ctx **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  auto it = g_ctx.find(dev);
+  if (it != g_ctx.end()) {
+    *out = it->second.get();
+    return FPSX_OK;
+  }
+  std::unique_ptr<Ctx> c(new Ctx());
+  c->device = dev;
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  if (khz <= 0) khz = 100000;
+  const char *sec = std::getenv("FPSX_EXCHANGE_TIMEOUT_S");
+  double secs = sec ? std::atof(sec) : 2.0;
+  if (!(secs > 0)) secs = 2.0;
+  c->spin_ticks = (uint64_t)((double)khz * 1000.0 * secs);
+  HIP_TRY(hipHostMalloc((void **)&c->host_status, sizeof(uint32_t), hipHostMallocDefault));
+  *out = c.get();
+  g_ctx[dev] = std::move(c);
+  return FPSX_OK;
+}
+
+int Ctx::max_blocks(const KernelEntry *k, size_t shmem) {
+  auto key = std::make_pair(k->fn, shmem);
+  auto it = occ.find(key);
+  if (it != occ.end()) return it->second;
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k->fn, k->block, shmem);
+  if (e != hipSuccess) nb = 0;
+  int total = nb * cus;
+  occ[key] = total;
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// Planner.  FPSX_PLAN overrides it (tests use this to reach every kernel):
+//   "resident:B:PPT"   register-resident kernel, grid = ceil(n / (B*PPT))
+//   "stream:B[:G]"     streaming kernel of the cloud's dimension (or generic)
+//   "generic:B[:G]"    generic any-dimension streaming kernel
+// Automatic choice (dimensions with resident shapes):
+//   n <= 16384: one workgroup, the smallest shape holding the cloud;
+//   otherwise: as many 4096-point workgroups as the cloud needs, growing the
+//   per-workgroup share (PPT 8, 16) when that exceeds the co-resident limit;
+//   beyond the register file: the streaming kernel.
+// ---------------------------------------------------------------------------
+static const KernelEntry *find_resident(int dim, int B, int ppt, int tie) {
+  size_t cnt;
+  const KernelEntry *t = resident_table(&cnt);
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].dim == dim && t[i].block == B && t[i].ppt == ppt && t[i].tie == tie) return &t[i];
+  return nullptr;
+}
+
+static const KernelEntry *find_stream(int dim, int B, int tie) {
+  size_t cnt;
+  const KernelEntry *t = stream_table(&cnt);
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].dim == dim && t[i].block == B && t[i].tie == tie) return &t[i];
+  return nullptr;
+}
+
+static int plan_stream(Ctx &c, size_t n, size_t dim, int tie, int B, size_t want_g, bool generic,
+                       Plan &p) {
+  const KernelEntry *k = generic ? nullptr : find_stream((int)dim, B, tie);
+  if (!k) k = find_stream(0, B, tie);
+  if (!k) return fail(FPSX_EINVAL, "no streaming kernel with block " + std::to_string(B));
+  p.kind = PlanKind::Stream;
+  p.k = k;
+  p.block = (uint32_t)B;
+  p.shmem = (k->dim == 0) ? dim * sizeof(float) : 0;
+  const size_t gmax = (size_t)c.max_blocks(k, p.shmem);
+  if (gmax == 0) return fail(FPSX_EHIP, "streaming kernel cannot be resident on this device");
+  size_t g = want_g ? want_g : (n + B - 1) / B;
+  if (g > gmax) g = gmax;
+  if (g < 1) g = 1;
+  size_t chunk = (n + g - 1) / g;
+  g = (n + chunk - 1) / chunk;  // no empty workgroup
+  p.grid = (uint32_t)g;
+  p.chunk = (uint32_t)chunk;
+  return FPSX_OK;
+}
+
+static int plan_resident_shape(Ctx &c, size_t n, const KernelEntry *k, Plan &p) {
+  const size_t per = (size_t)k->block * k->ppt;
+  const size_t g = (n + per - 1) / per;
+  if (g > 1 && g > (size_t)c.max_blocks(k, 0)) return -1;
+  p.kind = PlanKind::Resident;
+  p.k = k;
+  p.block = (uint32_t)k->block;
+  p.grid = (uint32_t)g;
+  p.chunk = (uint32_t)per;
+  p.shmem = 0;
+  return FPSX_OK;
+}
+
+int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
+  const char *env = std::getenv("FPSX_PLAN");
+  if (env && *env) {
+    std::string s(env);
+    int a = 0, b = 0, g = 0;
+    if (std::sscanf(s.c_str(), "resident:%d:%d", &a, &b) == 2) {
+      const KernelEntry *k = find_resident((int)dim, a, b, tie);
+      if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no resident kernel for dim " +
+                                           std::to_string(dim) + " shape " + s);
+      if (plan_resident_shape(c, n, k, p) != FPSX_OK)
+        return fail(FPSX_ETOOBIG, "FPSX_PLAN: cloud exceeds the co-resident grid of " + s);
+      return FPSX_OK;
+    }
+    if (s.rfind("stream:", 0) == 0 || s.rfind("generic:", 0) == 0) {
+      bool generic = s[0] == 'g';
+      int got = std::sscanf(s.c_str() + (generic ? 8 : 7), "%d:%d", &a, &g);
+      if (got < 1) return fail(FPSX_EINVAL, "FPSX_PLAN: bad value " + s);
+      return plan_stream(c, n, dim, tie, a, got == 2 ? (size_t)g : 0, generic, p);
+    }
+    return fail(FPSX_EINVAL, "FPSX_PLAN: bad value " + s);
+  }
+
+  // Single workgroup: the smallest resident shape that holds the cloud.
+  static const int kSmall[][2] = {{256, 1}, {256, 2}, {256, 4}, {1024, 1}, {1024, 2},
+                                  {1024, 4}, {1024, 8}, {1024, 16}};
+  if (n <= 16384) {
+    for (auto &sh : kSmall) {
+      if ((size_t)sh[0] * sh[1] < n) continue;
+      const KernelEntry *k = find_resident((int)dim, sh[0], sh[1], tie);
+      if (k && plan_resident_shape(c, n, k, p) == FPSX_OK) return FPSX_OK;
+    }
+  }
+  // Multi-workgroup resident: grow the per-workgroup share as needed.
+  static const int kMulti[][2] = {{1024, 4}, {1024, 8}, {1024, 16}, {256, 32}};
+  for (auto &sh : kMulti) {
+    const KernelEntry *k = find_resident((int)dim, sh[0], sh[1], tie);
+    if (k && plan_resident_shape(c, n, k, p) == FPSX_OK) return FPSX_OK;
+  }
+  return plan_stream(c, n, dim, tie, 1024, 0, false, p);
+}
+
+// ---------------------------------------------------------------------------
+// Launch on device-resident data.  Caller holds c.mu.
+// ---------------------------------------------------------------------------
+int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t ld, size_t n, size_t dim,
+               size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out, float init,
+               hipStream_t st) {
+  const size_t xwords = (size_t)2 * kSlotWords * p.grid;
+  HIP_TRY(c.xchg.reserve(xwords * sizeof(uint64_t)));
+  HIP_TRY(c.status.reserve(sizeof(uint64_t)));
+  launch_fill_u64((uint64_t *)c.xchg.p, xwords, 0, st);
+  launch_fill_u64((uint64_t *)c.status.p, 1, 0, st);
+  FpsParams a;
+  std::memset(&a, 0, sizeof(a));
+  a.pts = d_pts;
+  a.ld = ld;
+  a.n = (uint32_t)n;
+  a.m = (uint32_t)m;
+  a.starts = d_starts;
+  a.n_starts = (uint32_t)n_starts;
+  a.dim = (uint32_t)dim;
+  a.out = d_out;
+  a.xchg = (uint64_t *)c.xchg.p;
+  a.status = (uint32_t *)c.status.p;
+  a.init = init;
+  a.spin_ticks = c.spin_ticks;
+  a.chunk = p.chunk;
+  if (p.kind == PlanKind::Stream) {
+    HIP_TRY(c.dmin.reserve(n * sizeof(float)));
+    a.dmin = (float *)c.dmin.p;
+    launch_fill_f32(a.dmin, n, init, st);
+  }
+  void *args[] = {&a};
+  std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
+  if (c.timing) {
+    if (c.events_used == c.events.size()) {
+      std::pair<hipEvent_t, hipEvent_t> e2;
+      HIP_TRY(hipEventCreate(&e2.first));
+      HIP_TRY(hipEventCreate(&e2.second));
+      c.events.push_back(e2);
+    }
+    ev = &c.events[c.events_used++];
+    HIP_TRY(hipEventRecord(ev->first, st));
+  }
+  if (p.grid > 1) {
+    HIP_TRY(hipLaunchCooperativeKernel(p.k->fn, dim3(p.grid), dim3(p.block), args,
+                                       (unsigned)p.shmem, st));
+  } else {
+    HIP_TRY(hipLaunchKernel(p.k->fn, dim3(1), dim3(p.block), args, p.shmem, st));
+  }
+  if (ev) HIP_TRY(hipEventRecord(ev->second, st));
+  return FPSX_OK;
+}
+
+int status_code(uint32_t st) {
+  if (st & kStatusTimeout)
+    return fail(FPSX_ETIMEOUT, "cross-workgroup exchange timed out (workgroups not co-resident?)");
+  if (st & kStatusBadStart) return fail(FPSX_EBADSTART, "a start index is >= n_points");
+  return FPSX_OK;
+}
+
+// Host-array entry: upload, plan, launch, read back.  layout: FPSX_LAYOUT_*.
+int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
+             const uint64_t *starts, size_t n_starts, uint64_t *out, int tie, float init) {
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  Plan p;
+  rc = plan(c, n, dim, tie, p);
+  if (rc) return rc;
+  hipStream_t st = c.stream;
+  const size_t bytes = n * dim * sizeof(float);
+  HIP_TRY(c.pts.reserve(bytes));
+  HIP_TRY(c.starts.reserve(n_starts * sizeof(uint64_t)));
+  HIP_TRY(c.out.reserve(m * sizeof(uint64_t)));
+  if (layout == FPSX_LAYOUT_COLS) {
+    HIP_TRY(hipMemcpyAsync(c.pts.p, points, bytes, hipMemcpyHostToDevice, st));
+  } else {
+    HIP_TRY(c.rows.reserve(bytes));
+    HIP_TRY(hipMemcpyAsync(c.rows.p, points, bytes, hipMemcpyHostToDevice, st));
+    launch_rows_to_soa((const float *)c.rows.p, (float *)c.pts.p, (uint32_t)n, (uint32_t)dim, n,
+                       st);
+  }
+  HIP_TRY(hipMemcpyAsync(c.starts.p, starts, n_starts * sizeof(uint64_t), hipMemcpyHostToDevice,
+                         st));
+  rc = launch_fps(c, p, (const float *)c.pts.p, n, n, dim, m, (const uint64_t *)c.starts.p,
+                  n_starts, (uint64_t *)c.out.p, init, st);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, c.out.p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(c.host_status, c.status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return status_code(*c.host_status);
+}
+
+}  // namespace fpsx
+
+using namespace fpsx;
+
+FPSX_EXPORT const char *fpsx_last_error(void) { return t_err.c_str(); }
+
+FPSX_EXPORT const char *fpsx_version(void) { return FPSX_VERSION_STRING; }
+
+FPSX_EXPORT int fpsx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+FPSX_EXPORT int fpsx_fps_sampling(const float *points, size_t n_points, size_t dim, int layout,
+                                  size_t n_samples, const uint64_t *start_idx, size_t n_start,
+                                  uint64_t *out) {
+  t_err.clear();
+  if (dim == 0) return fail(FPSX_EBADDIM, "points must have at least one column");
+  if (!points || !start_idx || !out) return fail(FPSX_EINVAL, "null pointer argument");
+  if (layout != FPSX_LAYOUT_ROWS && layout != FPSX_LAYOUT_COLS)
+    return fail(FPSX_EINVAL, "unknown layout");
+  if (n_samples == 0 || n_samples > n_points)
+    return fail(FPSX_EINVAL, "n_samples must be in [1, n_points]: n_samples=" +
+                                 std::to_string(n_samples) + ", P=" + std::to_string(n_points));
+  if (n_start == 0 || n_start > n_samples)
+    return fail(FPSX_EINVAL, "the number of start indices must be in [1, n_samples]");
+  if (n_points >= (size_t(1) << 31)) return fail(FPSX_ETOOBIG, "n_points must be < 2^31");
+  for (size_t i = 0; i < n_start; ++i)
+    if (start_idx[i] >= n_points)
+      return fail(FPSX_EBADSTART, "start index " + std::to_string(start_idx[i]) +
+                                      " >= n_points " + std::to_string(n_points));
+  return fps_host(points, n_points, dim, layout, n_samples, start_idx, n_start, out,
+                  kTieHighest, INFINITY);
+}
+
+FPSX_EXPORT int fpsx_fps_sampling_device(const float *d_points, size_t ld, size_t n_points,
+                                         size_t dim, size_t n_samples, const uint64_t *d_start,
+                                         size_t n_start, uint64_t *d_out, int tie, float init,
+                                         void *stream) {
+  t_err.clear();
+  if (dim == 0) return fail(FPSX_EBADDIM, "dim must be >= 1");
+  if (!d_points || !d_start || !d_out) return fail(FPSX_EINVAL, "null pointer argument");
+  if (ld < n_points) return fail(FPSX_EINVAL, "ld must be >= n_points");
+  if (n_points == 0 || n_samples == 0) return fail(FPSX_EINVAL, "empty cloud or no samples");
+  if (n_start == 0 || n_start > n_samples)
+    return fail(FPSX_EINVAL, "the number of start indices must be in [1, n_samples]");
+  if (tie != FPSX_TIE_HIGHEST && tie != FPSX_TIE_LOWEST) return fail(FPSX_EINVAL, "bad tie rule");
+  if (n_points >= (size_t(1) << 31) || n_samples >= (size_t(1) << 32))
+    return fail(FPSX_ETOOBIG, "n_points must be < 2^31");
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  Plan p;
+  rc = plan(c, n_points, dim, tie, p);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+  return launch_fps(c, p, d_points, ld, n_points, dim, n_samples, d_start, n_start, d_out, init,
+                    st);
+}
+
+FPSX_EXPORT int fpsx_device_status(void *stream) {
+  t_err.clear();
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+  if (!c.status.p) return FPSX_OK;
+  HIP_TRY(hipMemcpyAsync(c.host_status, c.status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return status_code(*c.host_status);
+}
+
+FPSX_EXPORT int fpsx_plan_info(size_t n_points, size_t dim, int tie, int *kind, int *block,
+                               int *ppt, int *grid, int *chunk) {
+  t_err.clear();
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  Plan p;
+  rc = plan(c, n_points, dim, tie, p);
+  if (rc) return rc;
+  if (kind) *kind = (int)p.kind;
+  if (block) *block = (int)p.block;
+  if (ppt) *ppt = p.k->ppt;
+  if (grid) *grid = (int)p.grid;
+  if (chunk) *chunk = (int)p.chunk;
+  return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_timing(int enable) {
+  t_err.clear();
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(cp->mu);
+  cp->timing = enable != 0;
+  cp->events_used = 0;
+  return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_timing_read(double *total_ms, int *launches) {
+  t_err.clear();
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(cp->mu);
+  double sum = 0.0;
+  for (size_t i = 0; i < cp->events_used; ++i) {
+    HIP_TRY(hipEventSynchronize(cp->events[i].second));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, cp->events[i].first, cp->events[i].second));
+    sum += ms;
+  }
+  if (total_ms) *total_ms = sum;
+  if (launches) *launches = (int)cp->events_used;
+  cp->events_used = 0;
+  return FPSX_OK;
+}
+
+FPSX_EXPORT long fpsx_kdline_permutation(const float *raw_data, size_t n_points, size_t dim,
+                                         size_t height, uint64_t *perm, uint64_t *leaf_lo,
+                                         uint64_t *leaf_hi) {
+  t_err.clear();
+  if (dim == 0) return -(long)fail(FPSX_EBADDIM, "dim must be >= 1");
+  if (!raw_data || !perm || n_points == 0) return -(long)fail(FPSX_EINVAL, "bad arguments");
+  std::vector<uint64_t> pv, lo, hi;
+  if (kdline_permutation(raw_data, n_points, dim, height, pv, nullptr, &lo, &hi) != 0)
+    return -(long)fail(FPSX_EINVAL, "kdline permutation failed");
+  std::memcpy(perm, pv.data(), n_points * sizeof(uint64_t));
+  for (size_t k = 0; k < lo.size(); ++k) {
+    if (leaf_lo) leaf_lo[k] = lo[k];
+    if (leaf_hi) leaf_hi[k] = hi[k];
+  }
+  return (long)lo.size();
+}
+
+FPSX_EXPORT int bucket_fps_kdline(const float *raw_data, size_t n_points, size_t dim,
+                                  size_t n_samples, size_t start_idx, size_t height,
+                                  size_t *sampled_point_indices) {
+  t_err.clear();
+  if (dim == 0 || dim > FPSX_BUCKET_MAX_DIM)
+    return fail(FPSX_EBADDIM, "dim must be in [1, 8]");
+  if (start_idx >= n_points) return fail(FPSX_EBADSTART, "start_idx must be < n_points");
+  if (!raw_data || !sampled_point_indices) return fail(FPSX_EINVAL, "null pointer argument");
+  if (n_samples == 0) return fail(FPSX_EINVAL, "n_samples must be >= 1");
+  if (n_points >= (size_t(1) << 31)) return fail(FPSX_ETOOBIG, "n_points must be < 2^31");
+  std::vector<uint64_t> perm;
+  std::vector<float> rows;
+  if (kdline_permutation(raw_data, n_points, dim, height, perm, &rows, nullptr, nullptr) != 0)
+    return fail(FPSX_EINVAL, "kdline permutation failed");
+  std::vector<uint64_t> pos(n_samples);
+  uint64_t s0 = start_idx;
+  int rc = fps_host(rows.data(), n_points, dim, FPSX_LAYOUT_ROWS, n_samples, &s0, 1, pos.data(),
+                    kTieLowest, FLT_MAX);
+  if (rc) return rc;
+  for (size_t i = 0; i < n_samples; ++i) sampled_point_indices[i] = (size_t)perm[pos[i]];
+  return FPSX_OK;
+}

@@ -1,0 +1,65 @@
+// fpsx_internal.hpp -- host runtime internals shared by the C-ABI sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "fps_kernels.hpp"
+
+#define FPSX_VERSION_STRING "0.1.0"
+
+namespace fpsx {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t bytes);
+  ~DevBuf();
+};
+
+// One per device, created on first use and kept until the process ends.
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cus = 0;
+  uint64_t spin_ticks = 0;
+  uint32_t *host_status = nullptr;  // pinned
+  std::mutex mu;
+  DevBuf pts, rows, starts, out, xchg, status, dmin;
+  std::map<std::pair<const void *, size_t>, int> occ;  // co-resident workgroups per kernel
+  int max_blocks(const KernelEntry *k, size_t shmem);
+  // Optional per-launch timing of the FPS kernel (HIP events on its stream).
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t events_used = 0;
+};
+
+enum class PlanKind : int { Resident = 0, Stream = 1 };
+
+struct Plan {
+  PlanKind kind = PlanKind::Resident;
+  const KernelEntry *k = nullptr;
+  uint32_t block = 0;
+  uint32_t grid = 0;
+  uint32_t chunk = 0;  // points per workgroup
+  size_t shmem = 0;    // dynamic LDS bytes
+};
+
+int fail(int code, const std::string &msg);
+int get_ctx(Ctx **out);
+int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p);
+int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t ld, size_t n, size_t dim,
+               size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out, float init,
+               hipStream_t st);
+int status_code(uint32_t st);
+int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
+             const uint64_t *starts, size_t n_starts, uint64_t *out, int tie, float init);
+
+}  // namespace fpsx

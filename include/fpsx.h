@@ -1,0 +1,144 @@
+/*
+ * fpsx.h -- C ABI of the MI355X farthest-point-sampling library (libfpsx.so).
+ *
+ * Plain pointers and sizes only.  Every call is synchronous with respect to
+ * host memory unless its name ends in _device; those enqueue work on a HIP
+ * stream and return immediately.  All entry points are thread-safe: calls on
+ * one device are serialised by the library.
+ *
+ * Return codes follow the reference's bucket C API (src/wrapper.hpp:121-126):
+ * 0 success, 1 bad dimension, 2 bad start index; the codes from 3 up are new.
+ * fpsx_last_error() gives a message for the calling thread's last failure.
+ */
+#ifndef FPSX_H
+#define FPSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FPSX_OK 0
+#define FPSX_EBADDIM 1    /* dim == 0, or dim > FPSX_BUCKET_MAX_DIM for the bucket entry */
+#define FPSX_EBADSTART 2  /* a start index >= n_points */
+#define FPSX_EINVAL 3     /* other invalid argument (null pointer, n_samples, layout, ...) */
+#define FPSX_ETOOBIG 4    /* n_points >= 2^31 or a buffer larger than the device allows */
+#define FPSX_EHIP 5       /* HIP runtime error (no device, launch failure, ...) */
+#define FPSX_ENOMEM 6     /* device or host allocation failed */
+#define FPSX_ETIMEOUT 7   /* a cross-workgroup exchange did not complete (kernel gave up) */
+#define FPSX_EBUDGET 8    /* serving: request refused by the HBM budget */
+
+/* Point layouts accepted by the host entry points. */
+#define FPSX_LAYOUT_ROWS 0 /* row-major n x dim (C order): point i at p[i*dim + j] */
+#define FPSX_LAYOUT_COLS 1 /* column-major n x dim (Fortran order, SoA): p[j*n + i] */
+
+/* Tie rule among equal maximal distances. */
+#define FPSX_TIE_HIGHEST 0 /* highest index wins: src/lib.cpp:223-230 (`>=` scan) */
+#define FPSX_TIE_LOWEST 1  /* lowest index wins: src/_ext/KDLineTree.h:56-67 (`>`) */
+
+/* Largest dimension accepted by bucket_fps_kdline (src/wrapper.hpp:8-11). */
+#define FPSX_BUCKET_MAX_DIM 8
+
+/* Message describing the calling thread's last failure ("" if none). */
+const char *fpsx_last_error(void);
+
+/* Library version string. */
+const char *fpsx_version(void);
+
+/* Number of visible HIP devices (0 when none or the runtime is unavailable). */
+int fpsx_device_count(void);
+
+/*
+ * Synchronise `stream` (NULL: the library's stream for the current device) and
+ * report the status word of the last kernel enqueued by
+ * fpsx_fps_sampling_device: FPSX_OK, FPSX_ETIMEOUT or FPSX_EBADSTART.
+ */
+int fpsx_device_status(void *stream);
+
+/*
+ * The launch the planner picks for a cloud of n_points x dim under `tie`:
+ * kind 0 = register-resident, 1 = HBM-streaming; threads per workgroup,
+ * points per thread (resident), workgroups, points per workgroup.
+ */
+int fpsx_plan_info(size_t n_points, size_t dim, int tie, int *kind, int *block, int *ppt,
+                   int *grid, int *chunk);
+
+/*
+ * Kernel timing on the current device: fpsx_timing(1) makes every later FPS
+ * launch record a HIP event pair around the FPS kernel on the stream it is
+ * launched on (and forgets earlier records); fpsx_timing_read waits for the
+ * recorded launches, returns their summed kernel time and count, and forgets
+ * them.  fpsx_timing(0) stops recording.
+ */
+int fpsx_timing(int enable);
+int fpsx_timing_read(double *total_ms, int *launches);
+
+/*
+ * Vanilla farthest point sampling.
+ * Replaces fps_sampling (src/lib.cpp:188-246) when n_start == 1 and
+ * fps_sampling_multi_start_index (src/lib.cpp:111-186) when n_start > 1:
+ * start_idx[0..n_start) are picked first, in order, then each further pick is
+ * the point with the largest squared distance to the picked set, highest
+ * index on ties.  Squared distances are float32, summed in dimension order,
+ * no fused multiply-add.
+ *   points        host array, n_points x dim float32 in the given layout
+ *   start_idx     host array of n_start indices (1 <= n_start <= n_samples)
+ *   out           host array of n_samples indices (size_t in the reference)
+ * Errors: FPSX_EBADDIM (dim == 0), FPSX_EBADSTART, FPSX_EINVAL (n_samples == 0
+ * or > n_points, n_start == 0 or > n_samples, null pointer, bad layout).
+ */
+int fpsx_fps_sampling(const float *points, size_t n_points, size_t dim, int layout,
+                      size_t n_samples, const uint64_t *start_idx, size_t n_start,
+                      uint64_t *out);
+
+/*
+ * Same computation on device-resident data, enqueued on `stream`
+ * (a hipStream_t; NULL uses the library's own stream for the current device).
+ *   d_points  device SoA array: coordinate j of point i at d_points[j*ld + i], ld >= n_points
+ *   d_start   device array of n_start uint64 indices (values are not re-validated)
+ *   d_out     device array of n_samples uint64 indices
+ *   tie       FPSX_TIE_HIGHEST or FPSX_TIE_LOWEST
+ *   init      initial min-distance (+inf for fps_sampling, FLT_MAX for bucket FPS)
+ * The call returns once the work is enqueued; the library's scratch buffers
+ * stay in use until the stream reaches that point, so concurrent calls for
+ * one device must use the same stream.
+ */
+int fpsx_fps_sampling_device(const float *d_points, size_t ld, size_t n_points, size_t dim,
+                             size_t n_samples, const uint64_t *d_start, size_t n_start,
+                             uint64_t *d_out, int tie, float init, void *stream);
+
+/*
+ * Bucket FPS entry with the exact signature and return codes of the
+ * reference's C API bucket_fps_kdline (src/wrapper.hpp:118-132).
+ * Same picks as the reference: the KD-line tree of height `height` is built on
+ * the host exactly as src/_ext/KDTreeBase.h:84-207 builds it, the first pick
+ * is the point at position start_idx of the permuted order
+ * (src/wrapper.hpp:54-55), and exhaustive FPS runs on the GPU over the
+ * permuted order with FLT_MAX initial distances and lowest-position ties,
+ * which equals the reference's pruned bucket search.
+ *   raw_data  host row-major n_points x dim float32
+ *   sampled_point_indices  host array of n_samples size_t
+ * Returns 0, 1 (dim not in [1, 8]), 2 (start_idx >= n_points), or the codes
+ * from 3 up (n_samples == 0, GPU failures).  As in the reference, n_samples
+ * may exceed n_points; later picks then repeat the lowest tree position.
+ */
+int bucket_fps_kdline(const float *raw_data, size_t n_points, size_t dim, size_t n_samples,
+                      size_t start_idx, size_t height, size_t *sampled_point_indices);
+
+/*
+ * KD-line permutation alone (host): perm[pos] = original index of the point
+ * at position pos after the build of src/_ext/KDTreeBase.h:84-207 with leaves
+ * at depth `height` (src/_ext/KDLineTree.h:37-39).  leaf_lo/leaf_hi (may be
+ * NULL; capacity >= min(n_points, 2^height)) receive the bucket ranges.
+ * Returns the number of buckets (>= 1) or a negative FPSX_* code.
+ */
+long fpsx_kdline_permutation(const float *raw_data, size_t n_points, size_t dim, size_t height,
+                             uint64_t *perm, uint64_t *leaf_lo, uint64_t *leaf_hi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FPSX_H */

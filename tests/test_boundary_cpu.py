@@ -1,0 +1,179 @@
+"""The drop-in boundary without a GPU: exports, validation, start resolution.
+
+Everything here returns or raises before any device work, so it runs on the
+CPU.  Reference behaviour followed: src/fpsample/__init__.py:19-65, 174-206
+(asserts, dtype/layout coercion, RNG draw), src/lib.cpp:52-109, 249-270,
+522-579 (pybind11 validation and exception types), src/wrapper.hpp:118-132
+(C return codes).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import fpsample_amd as F
+from oracle import oracle as O
+from tests.capi import CApi, header_functions
+
+
+def test_library_exports_every_header_function(capi):
+    names = header_functions()
+    assert {"fpsx_fps_sampling", "bucket_fps_kdline", "fpsx_fps_sampling_device"} <= set(names)
+    for n in names:
+        assert hasattr(capi.L, n), n
+
+
+def test_version(capi):
+    assert capi.L.fpsx_version().decode() == F.__version__
+
+
+def test_bucket_signature_matches_reference_abi(capi):
+    # int bucket_fps_kdline(const float*, size_t, size_t, size_t, size_t, size_t, size_t*)
+    fn = capi.L.bucket_fps_kdline
+    assert fn.restype is ctypes.c_int and len(fn.argtypes) == 7
+
+
+def test_capi_validation_codes(capi):
+    pc = np.zeros((10, 3), np.float32)
+    assert capi.fps(pc, 3, 0, dim=0)[0] == 1            # no columns
+    assert capi.fps(pc, 3, 10)[0] == 2                  # start >= n
+    assert capi.fps(pc, 3, [0, 11])[0] == 2
+    assert capi.fps(pc, 0, 0)[0] == 3                   # no samples
+    assert capi.fps(pc, 11, 0)[0] == 3                  # more samples than points
+    assert capi.fps(pc, 2, [0, 1, 2])[0] == 3           # more starts than samples
+    assert capi.fps(pc, 3, 0, layout=7)[0] == 3
+    assert capi.kdline(np.zeros((10, 9), np.float32), 3, 0, 2)[0] == 1
+    assert capi.kdline(pc, 3, 0, 2, dim=0)[0] == 1
+    assert capi.kdline(pc, 3, 10, 2)[0] == 2
+    assert capi.kdline(pc, 0, 0, 2)[0] == 3
+    assert capi.last_error() != ""
+
+
+@pytest.mark.parametrize("d,h", [(1, 3), (2, 5), (3, 1), (3, 7), (3, 9), (8, 4)])
+def test_kdline_permutation_matches_oracle(capi, d, h):
+    rng = np.random.default_rng(d * 10 + h)
+    for pc in (rng.random((3000, d)), rng.integers(0, 4, (2000, d)), np.ones((300, d))):
+        pc = pc.astype(np.float32)
+        a = capi.kdline_perm(pc, h)
+        b = O.kdline_perm(pc, h)
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_kdline_permutation_golden_first_pick(capi, golden):
+    # The reference's first pick is the point at position start_idx of the tree order.
+    for name, g in golden.items():
+        if g["kind"] != "kdline":
+            continue
+        perm, _, _ = capi.kdline_perm(g["pc"], g["h"])
+        assert perm[int(g["start"][0])] == g["out"][0], name
+
+
+# ---------------------------------------------------------------- Python layer
+def test_asserts_match_reference():
+    pc = np.random.rand(10, 3)
+    cases = [
+        (lambda: F.fps_sampling(pc, 0), "n_samples should be >= 1"),
+        (lambda: F.fps_sampling(pc, 11), "n_pts should be >= n_samples"),
+        (lambda: F.fps_sampling(pc, 3, start_idx=10), "start_idx should be None or 0 <= start_idx < n_pts"),
+        (lambda: F.fps_sampling(pc, 3, start_idx=-1), "start_idx should be None or 0 <= start_idx < n_pts"),
+        (lambda: F.fps_sampling(pc, 2, start_idx=[0, 1, 2]), "len(start_idx) should be <= n_samples"),
+        (lambda: F.fps_sampling(pc, 3, start_idx=[0, 10]), "start_idx should be None or 0 <= start_idx < n_pts"),
+        (lambda: F.bucket_fps_kdline_sampling(pc, 3, 0), "h should be >= 1"),
+        (lambda: F.bucket_fps_kdline_sampling(pc, 3, 4), "2**h should be <= n_pts"),
+        (lambda: F.bucket_fps_kdline_sampling(pc, 3, 2, start_idx=10), "start_idx should be None or 0 <= start_idx < n_pts"),
+    ]
+    for fn, msg in cases:
+        with pytest.raises(AssertionError, match=msg.replace("*", r"\*").replace("(", r"\(").replace(")", r"\)")):
+            fn()
+    with pytest.raises(AssertionError):
+        F.fps_sampling(np.random.rand(10), 3)
+
+
+def test_start_idx_types():
+    assert F.get_start_idx(10, 3) == 3
+    a = F.get_start_idx(10, [1, 2])
+    assert isinstance(a, np.ndarray) and a.dtype == np.uint64 and a.tolist() == [1, 2]
+    with pytest.raises(ValueError, match="start_idx should be None, int or list"):
+        F.get_start_idx(10, (1, 2))
+    with pytest.raises(ValueError, match="start_idx should be None, int or list"):
+        F.fps_sampling(np.random.rand(10, 3), 3, start_idx=np.int64(2))
+    np.random.seed(42)
+    want = np.random.randint(low=0, high=4096)
+    np.random.seed(42)
+    assert F.get_start_idx(4096, None) == want
+
+
+def test_wrapper_hands_fortran_float32_and_one_rng_draw(monkeypatch):
+    seen = {}
+
+    def fake(pc, n, start):
+        seen.update(pc=pc, n=n, start=start)
+        return np.zeros(n, np.uint64)
+
+    monkeypatch.setattr(F, "_fps_sampling", fake)
+    pc = np.random.rand(50, 3)
+    np.random.seed(7)
+    F.fps_sampling(pc, 5)
+    np.random.seed(7)
+    assert seen["start"] == np.random.randint(0, 50)
+    assert seen["pc"].dtype == np.float32 and seen["pc"].flags.f_contiguous
+    np.testing.assert_array_equal(seen["pc"], pc.astype(np.float32))
+
+
+def test_pybind_validation_before_device():
+    from fpsample_amd import _fpsample as X
+
+    pc = np.asfortranarray(np.random.rand(10, 3).astype(np.float32))
+    with pytest.raises(ValueError, match="points must be a 2D array, but got shape 1"):
+        X._fps_sampling(np.zeros(10, np.float32), 3, 0)
+    with pytest.raises(ValueError, match="points must have at least one column"):
+        X._fps_sampling(np.zeros((10, 0), np.float32), 3, 0)
+    with pytest.raises(ValueError, match="n_samples must be less than the number of points: n_samples=11, P=10"):
+        X._fps_sampling(pc, 11, 0)
+    with pytest.raises(ValueError, match="start_idx must be less than the number of points: start_idx=10, P=10"):
+        X._fps_sampling(pc, 3, 10)
+    with pytest.raises(ValueError, match="The number of start indices must be less than or equal to n_samples: 3, n_samples=2"):
+        X._fps_sampling(pc, 2, np.array([0, 1, 2], np.uint64))
+    with pytest.raises(ValueError, match="All indices in start_idx must be less than the number of points: 12, P=10"):
+        X._fps_sampling(pc, 3, np.array([0, 12], np.uint64))
+    with pytest.raises(TypeError, match="start_idx must be int or 1D numpy array of size_t"):
+        X._fps_sampling(pc, 3, 1.5)
+    with pytest.raises(TypeError, match="start_idx must be int or 1D numpy array of size_t"):
+        X._fps_sampling(pc, 3, np.array([0, 1], np.int64))
+    with pytest.raises(ValueError, match="start_idx must not be an empty array"):
+        X._fps_sampling(pc, 3, np.array([], np.uint64))
+    assert X._fps_sampling(pc, 0, 0).shape == (0,)
+    with pytest.raises(NotImplementedError, match="Array of start indices not implemented yet"):
+        X._bucket_fps_kdline_sampling(pc, 3, 2, np.array([0], np.uint64))
+    with pytest.raises(ValueError, match="points must be a 2D float32 array"):
+        X._bucket_fps_kdline_sampling(np.zeros(10, np.float32), 3, 2, 0)
+    with pytest.raises(ValueError, match="start_idx out of range"):
+        X._bucket_fps_kdline_sampling(pc, 3, 2, 10)
+    with pytest.raises(ValueError, match=r"n_samples must be in \[1, num_points\]"):
+        X._bucket_fps_kdline_sampling(pc, 0, 2, 0)
+    with pytest.raises(ValueError, match="height must be >= 1"):
+        X._bucket_fps_kdline_sampling(pc, 3, 0, 0)
+
+
+def test_pybind_validation_matches_reference(reference):
+    """Same exception type and message as the compiled reference module."""
+    from fpsample_amd import _fpsample as X
+
+    pc = np.asfortranarray(np.random.rand(10, 3).astype(np.float32))
+    bad = [
+        (np.zeros(10, np.float32), 3, 0),
+        (np.zeros((10, 0), np.float32), 3, 0),
+        (pc, 11, 0),
+        (pc, 3, 10),
+        (pc, 2, np.array([0, 1, 2], np.uint64)),
+        (pc, 3, np.array([0, 12], np.uint64)),
+        (pc, 3, 1.5),
+    ]
+    for args in bad:
+        with pytest.raises(Exception) as ours:
+            X._fps_sampling(*args)
+        with pytest.raises(Exception) as theirs:
+            reference.module._fps_sampling(*args)
+        assert type(ours.value) is type(theirs.value)
+        assert str(ours.value) == str(theirs.value)

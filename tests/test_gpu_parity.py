@@ -1,0 +1,201 @@
+"""GPU parity: the HIP kernels against the oracle and the reference's golden vectors.
+
+Every comparison is bit-exact on indices (the picks are integers).  Sizes the
+oracle finishes in seconds are compared whole; the full benchmark sizes are
+checked through an oracle-compared prefix plus size-independent properties
+(forced-prefix resume reproduces the run, picks distinct while distances are
+positive).  FPSX_PLAN forces each kernel shape so every instantiation family
+the planner can pick is exercised.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.capi import COLS, ROWS
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture
+def plan_env(monkeypatch):
+    def set_plan(p):
+        if p is None:
+            monkeypatch.delenv("FPSX_PLAN", raising=False)
+        else:
+            monkeypatch.setenv("FPSX_PLAN", p)
+    yield set_plan
+    monkeypatch.delenv("FPSX_PLAN", raising=False)
+
+
+def run_fps(gpu, pc, m, starts, layout=COLS):
+    rc, out = gpu.fps(pc, m, starts, layout=layout)
+    assert rc == 0, f"rc={rc}: {gpu.last_error()}"
+    return out
+
+
+def test_golden_through_python_api(gpu, golden):
+    import fpsample_amd as F
+
+    for name, g in golden.items():
+        log(f"golden {name}")
+        st = g["start"]
+        if g["kind"] == "fps":
+            arg = int(st[0]) if st.size == 1 else [int(x) for x in st]
+            out = F.fps_sampling(g["pc"], g["m"], start_idx=arg)
+        else:
+            out = F.bucket_fps_kdline_sampling(g["pc"], g["m"], g["h"], start_idx=int(st[0]))
+        assert out.dtype == np.uint64
+        np.testing.assert_array_equal(out, g["out"], err_msg=name)
+
+
+def test_golden_through_capi_rows(gpu, golden):
+    for name, g in golden.items():
+        if g["kind"] != "fps":
+            continue
+        log(f"golden rows {name}")
+        np.testing.assert_array_equal(run_fps(gpu, g["pc"], g["m"], g["start"], layout=ROWS),
+                                      g["out"], err_msg=name)
+
+
+def _clouds():
+    rng = np.random.default_rng(2024)
+    yield "uniform", rng.random((5000, 3)).astype(np.float32)
+    yield "lattice_dups", np.repeat(rng.integers(0, 5, (1500, 3)), 4, axis=0).astype(np.float32)
+    yield "gauss", (rng.standard_normal((3333, 3)) * 100).astype(np.float32)
+
+
+RESIDENT_SHAPES = ["resident:256:1", "resident:256:4", "resident:256:16", "resident:256:32",
+                   "resident:512:8", "resident:1024:1", "resident:1024:4", "resident:1024:16"]
+
+
+@pytest.mark.parametrize("shape", RESIDENT_SHAPES)
+def test_resident_shapes(gpu, plan_env, shape):
+    # Small per-workgroup shares force multi-workgroup grids (cross-WG exchange).
+    plan_env(shape)
+    for name, pc in _clouds():
+        for starts in (0, [len(pc) - 1, 5, 9]):
+            log(f"{shape} {name} starts={starts}")
+            want = O.fps(pc, 700, starts)
+            np.testing.assert_array_equal(run_fps(gpu, pc, 700, starts), want,
+                                          err_msg=f"{shape} {name}")
+
+
+@pytest.mark.parametrize("shape", ["stream:1024", "stream:256", "stream:1024:3", "stream:1024:40",
+                                   "generic:256", "generic:1024:7"])
+def test_stream_shapes(gpu, plan_env, shape):
+    plan_env(shape)
+    for name, pc in _clouds():
+        log(f"{shape} {name}")
+        want = O.fps(pc, 500, [7, 1])
+        np.testing.assert_array_equal(run_fps(gpu, pc, 500, [7, 1]), want,
+                                      err_msg=f"{shape} {name}")
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 5, 6, 7, 8, 9, 13, 16, 40])
+def test_dimensions_auto_plan(gpu, d):
+    rng = np.random.default_rng(d)
+    for n in (37, 1000, 20000):
+        pc = rng.standard_normal((n, d)).astype(np.float32)
+        m = min(n, 300)
+        log(f"dim {d} n {n} plan {gpu.plan_info(n, d)}")
+        np.testing.assert_array_equal(run_fps(gpu, pc, m, 3 % n), O.fps(pc, m, 3 % n),
+                                      err_msg=f"d={d} n={n}")
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 5, 8])
+def test_dimensions_forced_stream(gpu, plan_env, d):
+    plan_env("stream:1024:5")
+    rng = np.random.default_rng(50 + d)
+    pc = rng.integers(0, 4, (3000, d)).astype(np.float32)
+    np.testing.assert_array_equal(run_fps(gpu, pc, 600, 0), O.fps(pc, 600, 0))
+
+
+def test_edge_sizes(gpu):
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 3, 63, 64, 65, 255, 256, 257, 1023, 1024, 1025, 4097, 16384, 16385, 40000):
+        pc = rng.random((n, 3)).astype(np.float32)
+        for m in sorted({k for k in (1, 2, min(n, 5), min(n, 300)) if k <= n}):
+            log(f"edge n={n} m={m}")
+            np.testing.assert_array_equal(run_fps(gpu, pc, m, n - 1), O.fps(pc, m, n - 1),
+                                          err_msg=f"n={n} m={m}")
+
+
+def test_all_points_then_duplicates(gpu):
+    # More samples than distinct points: the reference repeats the highest index.
+    pc = np.repeat(np.array([[0, 0, 0], [1, 1, 1], [2, 0, 1]], np.float32), 5, axis=0)
+    np.testing.assert_array_equal(run_fps(gpu, pc, 15, 0), O.fps(pc, 15, 0))
+
+
+def test_special_values(gpu):
+    rng = np.random.default_rng(12)
+    pc = (rng.random((2000, 3)) * 4e19).astype(np.float32)   # squares overflow to +inf
+    pc[[5, 700]] = np.nan
+    pc[9] = np.inf
+    np.testing.assert_array_equal(run_fps(gpu, pc, 300, 0), O.fps(pc, 300, 0))
+    tiny = (rng.random((2000, 3)) * 1e-21).astype(np.float32)  # denormal squares
+    np.testing.assert_array_equal(run_fps(gpu, tiny, 300, 0), O.fps(tiny, 300, 0))
+
+
+def test_kdline_capi_against_oracle(gpu):
+    rng = np.random.default_rng(13)
+    for pc, h in ((rng.random((20000, 3)).astype(np.float32), 9),
+                  (rng.integers(0, 6, (5000, 3)).astype(np.float32), 5),
+                  (rng.random((3000, 7)).astype(np.float32), 3)):
+        for s in (0, len(pc) // 2):
+            rc, out = gpu.kdline(pc, 400, s, h)
+            assert rc == 0, gpu.last_error()
+            np.testing.assert_array_equal(out, O.kdline_quickfps(pc, 400, h, s))
+
+
+def test_kdline_capi_error_codes(gpu):
+    pc = np.zeros((10, 9), np.float32)
+    assert gpu.kdline(pc, 3, 0, 2)[0] == 1
+    assert gpu.kdline(pc[:, :3], 3, 10, 2)[0] == 2
+
+
+def c2_cloud(n=1 << 20, seed=1):
+    rng = np.random.default_rng(seed)
+    centers = rng.random((64, 3))
+    lab = rng.integers(0, 64, n)
+    return (centers[lab] + 0.02 * rng.standard_normal((n, 3))).astype(np.float32)
+
+
+def c5_cloud(n=1 << 24, seed=4):
+    rng = np.random.default_rng(seed)
+    distinct = n * 3 // 4
+    sites = rng.choice(1 << 24, distinct, replace=False)
+    copies = sites[rng.integers(0, distinct, n - distinct)]
+    codes = np.concatenate([sites, copies])
+    rng.shuffle(codes)
+    return np.stack([(codes >> 16) & 255, (codes >> 8) & 255, codes & 255], axis=1).astype(np.float32)
+
+
+def test_large_single_cloud_properties(gpu):
+    pc = c2_cloud()
+    m = 16384
+    log(f"C2 plan {gpu.plan_info(len(pc), 3)}")
+    out = run_fps(gpu, pc, m, 0)
+    np.testing.assert_array_equal(out[:128], O.fps(pc, 128, 0))
+    assert len(np.unique(out)) == m
+    k = 9000
+    resumed = run_fps(gpu, pc, m, out[:k])
+    np.testing.assert_array_equal(resumed, out)
+
+
+def test_lattice_ties_streaming_size(gpu):
+    pc = c5_cloud()
+    m = 4096
+    log(f"C5 plan {gpu.plan_info(len(pc), 3)}")
+    out = run_fps(gpu, pc, m, 0)
+    np.testing.assert_array_equal(out[:24], O.fps(pc, 24, 0))
+    assert len(np.unique(out)) == m
+    resumed = run_fps(gpu, pc, m, out[:3000])
+    np.testing.assert_array_equal(resumed, out)

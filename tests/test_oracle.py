@@ -1,0 +1,183 @@
+"""Pins the CPU oracle (oracle/fps_oracle.c, oracle/oracle.py) -- CPU only.
+
+Each oracle function is checked against something it does not share code with:
+golden vectors written by the compiled reference, hand-derived known answers,
+the definition of FPS evaluated by brute force, the exactness of bucket
+pruning, and (where oracle/_ref is built) the reference itself on random
+inputs.  A planted mistake (the opposite tie rule) must fail the known answers.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def test_golden_fps(golden):
+    n = 0
+    for name, g in golden.items():
+        if g["kind"] != "fps":
+            continue
+        out = O.fps(g["pc"], g["m"], g["start"])
+        np.testing.assert_array_equal(out, g["out"], err_msg=name)
+        n += 1
+    assert n >= 15
+
+
+def test_golden_kdline(golden):
+    n = 0
+    for name, g in golden.items():
+        if g["kind"] != "kdline":
+            continue
+        s = int(g["start"][0])
+        np.testing.assert_array_equal(O.kdline_quickfps(g["pc"], g["m"], g["h"], s), g["out"],
+                                      err_msg=name)
+        np.testing.assert_array_equal(O.kdline_exhaustive(g["pc"], g["m"], g["h"], s), g["out"],
+                                      err_msg=name)
+        n += 1
+    assert n >= 15
+
+
+# Known answers derived by hand from src/lib.cpp:211-238 (`>=` argmax).
+KATS = [
+    (np.arange(5, dtype=np.float32)[:, None], 5, 0, [0, 4, 2, 3, 1]),
+    (np.array([[0], [0], [1], [1]], np.float32), 4, 0, [0, 3, 3, 3]),
+    (np.zeros((6, 3), np.float32), 6, 2, [2, 5, 5, 5, 5, 5]),
+    (np.array([[0, 0], [3, 4], [0, 4], [3, 0]], np.float32), 4, 0, [0, 1, 3, 2]),
+]
+
+
+@pytest.mark.parametrize("pc,m,s,want", KATS)
+def test_known_answers(pc, m, s, want):
+    assert O.fps(pc, m, s).tolist() == want
+    assert O.fps_numpy(pc, m, s).tolist() == want
+
+
+def test_planted_tie_mistake_is_caught():
+    # The lowest-index rule is a plausible mistake; the known answers reject it.
+    pc, m, s, want = KATS[1]
+    assert O.fps_ex(pc, m, s, lowest=True).tolist() != want
+    pc, m, s, want = KATS[0]
+    assert O.fps_ex(pc, m, s, lowest=True).tolist() != want
+
+
+def _brute_force(pc, m, starts, lowest=False):
+    """FPS by definition: pick = argmax_i min_{p in picked} d(i, p), float32 d."""
+    pc = np.asarray(pc, np.float32)
+    n, d = pc.shape
+    picked = [int(s) for s in starts[:1]]
+    rest = [int(s) for s in starts[1:]]
+    while len(picked) < m:
+        if rest:
+            picked.append(rest.pop(0))
+            continue
+        best, bi = None, None
+        for i in range(n):
+            md = np.float32(np.inf)
+            for p in picked:
+                acc = np.float32(0)
+                for j in range(d):
+                    t = np.float32(pc[i, j] - pc[p, j])
+                    acc = np.float32(acc + np.float32(t * t))
+                if acc < md:
+                    md = acc
+            if best is None or md > best or (md == best and not lowest):
+                best, bi = md, i
+        picked.append(bi)
+    return picked
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_brute_force_tiny(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 9))
+    d = int(rng.integers(1, 4))
+    # small integer grid: many exact ties
+    pc = rng.integers(0, 3, (n, d)).astype(np.float32)
+    m = int(rng.integers(1, n + 1))
+    k = int(rng.integers(1, m + 1))
+    starts = rng.integers(0, n, k).tolist()
+    assert O.fps(pc, m, starts).tolist() == _brute_force(pc, m, starts)
+    assert O.fps_ex(pc, m, starts, lowest=True).tolist() == _brute_force(pc, m, starts, True)
+
+
+def test_numpy_and_c_agree_on_ties():
+    rng = np.random.default_rng(7)
+    pc = rng.integers(0, 4, (400, 3)).astype(np.float32)
+    for s in (0, 399, [3, 5, 7]):
+        np.testing.assert_array_equal(O.fps(pc, 200, s), O.fps_numpy(pc, 200, s))
+        np.testing.assert_array_equal(O.fps_ex(pc, 200, s, lowest=True),
+                                      O.fps_numpy(pc, 200, s, lowest=True))
+
+
+def test_resume_from_prefix():
+    # Forced picks replay a prefix: the continuation equals the uninterrupted run.
+    rng = np.random.default_rng(3)
+    pc = rng.random((3000, 3)).astype(np.float32)
+    full = O.fps(pc, 500, 11)
+    for k in (1, 2, 37, 499, 500):
+        np.testing.assert_array_equal(O.fps(pc, 500, full[:k]), full)
+
+
+def test_picked_distances_non_increasing():
+    rng = np.random.default_rng(4)
+    pc = rng.standard_normal((2000, 4)).astype(np.float32)
+    out, dist = O.fps_ex(pc, 400, 0, with_dist=True)
+    assert np.all(np.diff(dist[1:]) <= 0)
+    assert len(set(out.tolist())) == 400
+
+
+def test_kdline_perm_invariants():
+    rng = np.random.default_rng(6)
+    pc = rng.random((1000, 3)).astype(np.float32)
+    for h in (1, 3, 6):
+        perm, lo, hi = O.kdline_perm(pc, h)
+        assert sorted(perm.tolist()) == list(range(1000))
+        assert len(lo) == 2 ** h
+        assert lo[0] == 0 and hi[-1] == 1000 and np.all(lo[1:] == hi[:-1])
+
+
+@pytest.mark.parametrize("h", [1, 2, 4, 6])
+def test_kdline_pruning_is_exact(h):
+    # Lazy bucket updates == exhaustive search over the tree order (ties included).
+    rng = np.random.default_rng(h)
+    pcs = [rng.integers(0, 5, (1500, 3)).astype(np.float32),
+           rng.random((1200, 2)).astype(np.float32),
+           np.repeat(rng.integers(0, 3, (200, 3)), 5, axis=0).astype(np.float32)]
+    for pc in pcs:
+        for s in (0, 5, len(pc) - 1):
+            np.testing.assert_array_equal(O.kdline_quickfps(pc, 600, h, s),
+                                          O.kdline_exhaustive(pc, 600, h, s))
+
+
+def test_kdline_tie_free_equals_vanilla_tail():
+    # On tie-free data only the start and the initial distance differ.
+    rng = np.random.default_rng(8)
+    pc = rng.random((2000, 3)).astype(np.float32)
+    kd = O.kdline_quickfps(pc, 300, 5, 9)
+    van = O.fps(pc, 300, int(kd[0]))
+    np.testing.assert_array_equal(kd, van)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_against_reference_random(reference, seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(50, 3000))
+    d = int(rng.integers(1, 9))
+    pc = (rng.integers(0, 4, (n, d)) if seed % 2 else rng.standard_normal((n, d)))
+    pc = pc.astype(np.float32)
+    m = int(rng.integers(1, min(n, 400) + 1))
+    s = int(rng.integers(0, n))
+    np.testing.assert_array_equal(O.fps(pc, m, s), reference.fps_sampling(pc, m, s))
+    h = int(rng.integers(1, max(2, int(np.log2(n)))))
+    np.testing.assert_array_equal(O.kdline_quickfps(pc, m, h, s),
+                                  reference.bucket_fps_kdline(pc, m, h, s))
+    starts = rng.integers(0, n, int(rng.integers(1, m + 1))).tolist()
+    np.testing.assert_array_equal(O.fps(pc, m, starts), reference.fps_sampling(pc, m, starts))
+
+
+def test_rng_draw_is_numpy_legacy():
+    # start_idx=None draws np.random.randint(0, n) from the global legacy RNG
+    # (src/fpsample/__init__.py:22); the legacy MT19937 stream is frozen across
+    # numpy versions, so the value after seed(42) is fixed.
+    np.random.seed(42)
+    assert np.random.randint(low=0, high=4096) == 3174
