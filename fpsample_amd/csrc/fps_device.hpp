@@ -4,10 +4,14 @@
 // its bit pattern read as int32 orders exactly like the float.  Padding lanes
 // carry -1.0f, whose int32 pattern is negative and loses to every real value.
 //
-// Exchange area (device memory, uint64 words):  word(buf, w, g) at
-//   xchg[(buf * kSlotWords + w) * G + g]
-// holds word w of workgroup g's candidate for exchange buffer buf (0/1):
-//   w = 0 value bits, w = 1 point index, w = 2 + j coordinate j bits.
+// Exchange area (device memory, uint64 words).  Word k of workgroup g's
+// candidate for exchange buffer buf, replica r, lives at
+//   xchg[((buf * R + r) * kSlotWords + k) * kPageWords + g]
+// so every word array sits in its own 4 KiB page: the polls of all
+// workgroups spread over many memory channels instead of hammering the one
+// or two pages a packed layout would occupy.  A workgroup writes its
+// candidate to every replica and reads replica (g mod R).
+//   k = 0 value bits, k = 1 point index, k = 2 + j coordinate j bits.
 // Each word is (payload << 32) | tag with tag = exchange number (1, 2, ...).
 // Every 64-bit word carries its own tag, so a reader never needs ordering
 // between words: it polls until every word it needs shows the tag it expects.
@@ -25,6 +29,8 @@ constexpr int kTieHighest = 0;
 constexpr int kTieLowest = 1;
 constexpr int kSlotWords = 16;  // value, index, up to 14 coordinates
 constexpr int kMaxXchgDim = kSlotWords - 2;
+constexpr int kPageWords = 512;  // one 4 KiB page per word array: G <= 512
+constexpr int kMaxReplicas = 8;
 
 // Status bits written by kernels (device word, cleared before each launch).
 constexpr uint32_t kStatusTimeout = 1u;
@@ -35,30 +41,32 @@ constexpr uint32_t kStatusTimeout = 1u;
 //   quad_perm[1,0,3,2], quad_perm[2,3,0,1]  -> max within each quad
 //   row_shr:4, row_shr:8                    -> lane 15 of a row holds the row max
 //   row_bcast:15 (rows 1,3), row_bcast:31 (rows 2,3) -> lane 63 holds the wave max
-// Lanes a DPP move cannot source from keep their own value (old = src), which
-// max tolerates because it is idempotent.
 // ---------------------------------------------------------------------------
-#define FPSX_DPP(v, ctrl, rmask) __builtin_amdgcn_update_dpp((v), (v), (ctrl), (rmask), 0xf, false)
+// old = the identity of max, so lanes a DPP move cannot source from (and rows
+// the row mask leaves out) contribute nothing; this form lets the compiler
+// fold each move into the max instruction (v_max_*_dpp).
+#define FPSX_DPP(id, v, ctrl, rmask) __builtin_amdgcn_update_dpp((id), (v), (ctrl), (rmask), 0xf, false)
 
 __device__ __forceinline__ int wave_max_i32(int v) {
-  v = max(v, FPSX_DPP(v, 0xb1, 0xf));
-  v = max(v, FPSX_DPP(v, 0x4e, 0xf));
-  v = max(v, FPSX_DPP(v, 0x114, 0xf));
-  v = max(v, FPSX_DPP(v, 0x118, 0xf));
-  v = max(v, FPSX_DPP(v, 0x142, 0xa));
-  v = max(v, FPSX_DPP(v, 0x143, 0xc));
+  constexpr int kId = INT_MIN;
+  v = max(v, FPSX_DPP(kId, v, 0xb1, 0xf));   // quad_perm [1,0,3,2]
+  v = max(v, FPSX_DPP(kId, v, 0x4e, 0xf));   // quad_perm [2,3,0,1]
+  v = max(v, FPSX_DPP(kId, v, 0x114, 0xf));  // row_shr:4
+  v = max(v, FPSX_DPP(kId, v, 0x118, 0xf));  // row_shr:8
+  v = max(v, FPSX_DPP(kId, v, 0x142, 0xa));  // row_bcast:15
+  v = max(v, FPSX_DPP(kId, v, 0x143, 0xc));  // row_bcast:31
   return __builtin_amdgcn_readlane(v, 63);
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t u) {
-  int v = (int)u;
-  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0xb1, 0xf));
-  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x4e, 0xf));
-  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x114, 0xf));
-  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x118, 0xf));
-  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x142, 0xa));
-  v = (int)max((uint32_t)v, (uint32_t)FPSX_DPP(v, 0x143, 0xc));
-  return (uint32_t)__builtin_amdgcn_readlane(v, 63);
+  uint32_t v = u;
+  v = max(v, (uint32_t)FPSX_DPP(0, (int)v, 0xb1, 0xf));
+  v = max(v, (uint32_t)FPSX_DPP(0, (int)v, 0x4e, 0xf));
+  v = max(v, (uint32_t)FPSX_DPP(0, (int)v, 0x114, 0xf));
+  v = max(v, (uint32_t)FPSX_DPP(0, (int)v, 0x118, 0xf));
+  v = max(v, (uint32_t)FPSX_DPP(0, (int)v, 0x142, 0xa));
+  v = max(v, (uint32_t)FPSX_DPP(0, (int)v, 0x143, 0xc));
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 #undef FPSX_DPP
@@ -137,5 +145,13 @@ __device__ __forceinline__ uint64_t xchg_load(const uint64_t *p) {
 }
 
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for
+// every outstanding global store of the wave (vmcnt counts stores on gfx9),
+// which would put a memory round trip on every pass; the kernels never need
+// global memory ordered across this barrier.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 
 }  // namespace fpsx

@@ -19,65 +19,94 @@
 //                                 pass (coalesced SoA loads), for clouds larger
 //                                 than the register file; D == 0 is the generic
 //                                 any-dimension form (pivot re-read from HBM).
-// Both need all G workgroups co-resident (cooperative launch when G > 1).
+// Both need all G workgroups co-resident (cooperative launch when G > 1),
+// G <= B (the exchange polls with one lane per workgroup over ceil(G/64)
+// waves) and G <= 512 (one exchange page per word array).
 #include "fps_device.hpp"
 #include "fps_kernels.hpp"
 
 namespace fpsx {
 
 // ---------------------------------------------------------------------------
-// Cross-workgroup exchange, run by wave 0 of every workgroup.
-// Lane w < NW holds word w of this workgroup's candidate in `mine`; on return
-// lane w < NW holds word w of the winning candidate in `win`.  Returns false
-// when some candidate did not arrive before `deadline` (realtime ticks).
+// Cross-workgroup step, entered by every wave once the workgroup's candidate
+// is known (word k of it in cand[k][gl], k < NW).  Wave 0 publishes the
+// candidate; waves 0..R-1 (R = ceil(G/64), host guarantees R <= W) each poll
+// 64 slots, one slot per lane with all NW words loaded at once, reduce them
+// and leave their winner in xw[k][wave] (xw[NW][wave] = arrived in time);
+// after one LDS barrier every wave reduces the R winners.  On return win[k]
+// holds word k of the grid's winner in every lane.  Returns false when some
+// candidate did not arrive within a.spin_ticks.
 // ---------------------------------------------------------------------------
-template <int TIE, int NW>
-__device__ __forceinline__ bool exchange_wave0(uint64_t *__restrict__ xchg, uint32_t G, uint32_t g,
-                                               uint32_t e, int mine, int &win, uint64_t deadline) {
+template <int TIE, int NW, int W>
+__device__ __forceinline__ bool grid_exchange(const FpsParams &a, int (*cand)[W], int (*xw)[W],
+                                              int gl, uint32_t e, int (&win)[NW]) {
   const int lane = threadIdx.x & 63;
-  uint64_t *buf = xchg + (size_t)(e & 1u) * kSlotWords * G;
-  if (lane < NW)
-    xchg_store(buf + (size_t)lane * G + g, ((uint64_t)(uint32_t)mine << 32) | e);
-
-  int bv = INT_MIN;
-  uint32_t bi = 0;
-  int bw[NW];
+  const int wave = threadIdx.x >> 6;
+  const uint32_t G = gridDim.x;
+  const uint32_t g = blockIdx.x;
+  const int R = (int)((G + 63u) >> 6);
+  const int K = min((int)a.replicas, 64 / NW);  // replicas this candidate size allows
+  uint64_t *bufs = a.xchg + (size_t)(e & 1u) * a.replicas * kSlotWords * kPageWords;
+  if (wave == 0 && lane < K * NW) {
+    const int rep = lane / NW;
+    const int k = lane - rep * NW;
+    xchg_store(bufs + ((size_t)rep * kSlotWords + k) * kPageWords + g,
+               ((uint64_t)(uint32_t)cand[k][gl] << 32) | e);
+  }
+  uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * kPageWords;
+  if (wave < R) {
+    const uint32_t q = (uint32_t)(wave * 64 + lane);
+    int cv = INT_MIN;
+    uint32_t ci = 0;
+    int cw[NW];
 #pragma unroll
-  for (int k = 0; k < NW; ++k) bw[k] = 0;
-  bool ok = true;
-  for (uint32_t q = lane; q < G; q += 64) {
-    uint64_t w[NW];
-    for (;;) {
-      bool all = true;
+    for (int k = 0; k < NW; ++k) cw[k] = 0;
+    bool ok = true;
+    if (q < G) {
+      uint64_t w[NW];
+      const uint64_t deadline = now_ticks() + a.spin_ticks;
+      for (;;) {
+        bool all = true;
 #pragma unroll
-      for (int k = 0; k < NW; ++k) {
-        w[k] = xchg_load(buf + (size_t)k * G + q);
-        all = all && ((uint32_t)w[k] == e);
+        for (int k = 0; k < NW; ++k) w[k] = xchg_load(buf + (size_t)k * kPageWords + q);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) all = all && ((uint32_t)w[k] == e);
+        if (all) break;
+        if (now_ticks() > deadline) {
+          ok = false;
+          break;
+        }
+        for (uint32_t z = 0; z < a.backoff; ++z) __builtin_amdgcn_s_sleep(1);
       }
-      if (all) break;
-      if (now_ticks() > deadline) {
-        ok = false;
-        break;
+      if (ok) {
+        cv = (int)(w[0] >> 32);
+        ci = (uint32_t)(w[1] >> 32);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) cw[k] = (int)(w[k] >> 32);
       }
     }
-    const int v = (int)(w[0] >> 32);
-    const uint32_t i = (uint32_t)(w[1] >> 32);
-    if (ok && beats<TIE>(v, i, bv, bi)) {
-      bv = v;
-      bi = i;
+    int xv;
+    const int xl = wave_argmax_lane<TIE>(cv, ci, xv);
+    const bool wave_ok = __ballot(!ok) == 0;
+    int word = 0;
 #pragma unroll
-      for (int k = 0; k < NW; ++k) bw[k] = (int)(w[k] >> 32);
+    for (int k = 0; k < NW; ++k) {
+      const int x = readlane_i(cw[k], xl);
+      if (lane == k) word = x;
     }
+    if (lane < NW) xw[lane][wave] = word;
+    if (lane == NW) xw[NW][wave] = wave_ok ? 1 : 0;
   }
-  int wv;
-  const int wl = wave_argmax_lane<TIE>(bv, bi, wv);
-  win = 0;
+  lds_barrier();
+  bool ok_all = true;
+  for (int r = 0; r < R; ++r) ok_all = ok_all && (xw[NW][r] != 0);
+  const int v3 = (lane < R) ? xw[0][lane] : INT_MIN;
+  const uint32_t i3 = (lane < R) ? (uint32_t)xw[1][lane] : 0u;
+  int gv;
+  const int wl = wave_argmax_lane<TIE>(v3, i3, gv);
 #pragma unroll
-  for (int k = 0; k < NW; ++k) {
-    const int x = readlane_i(bw[k], wl);
-    if (lane == k) win = x;
-  }
-  return __ballot(!ok) == 0;
+  for (int k = 0; k < NW; ++k) win[k] = xw[k][wl];
+  return ok_all;
 }
 
 // Coordinates of a forced pick (multi-start prefix).  An index outside the
@@ -105,7 +134,7 @@ __global__ void __launch_bounds__(B) fps_resident(FpsParams a) {
   constexpr int W = B / 64;
   constexpr int NW = 2 + D;
   __shared__ int s_cand[2][NW][W];
-  __shared__ int s_piv[2][NW + 1];
+  __shared__ int s_xw[2][NW + 1][W];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -144,7 +173,7 @@ __global__ void __launch_bounds__(B) fps_resident(FpsParams a) {
     }
     if (s < a.n_starts) {
       pidx = load_forced<D>(a, s, pv);
-      if (g == 0 && tid == 0) a.out[s] = pidx;
+      if (g == 0 && tid == B - 64) a.out[s] = pidx;
       continue;
     }
     ++e;
@@ -170,7 +199,7 @@ __global__ void __launch_bounds__(B) fps_resident(FpsParams a) {
 #pragma unroll
       for (int j = 0; j < D; ++j) s_cand[par][2 + j][wave] = __float_as_int(wc[j]);
     }
-    __syncthreads();
+    lds_barrier();
 
     // Workgroup argmax (every wave reduces the W wave candidates).
     const int v2 = (lane < W) ? s_cand[par][0][lane] : INT_MIN;
@@ -183,24 +212,18 @@ __global__ void __launch_bounds__(B) fps_resident(FpsParams a) {
 #pragma unroll
       for (int j = 0; j < D; ++j) pv[j] = __int_as_float(s_cand[par][2 + j][gl]);
     } else {
-      if (wave == 0) {
-        const int mine = (lane < NW) ? s_cand[par][lane][gl] : 0;
-        int win;
-        const bool ok = exchange_wave0<TIE, NW>(a.xchg, G, g, e, mine, win,
-                                                now_ticks() + a.spin_ticks);
-        if (lane < NW) s_piv[par][lane] = win;
-        if (lane == 0) s_piv[par][NW] = ok ? 1 : 0;
-      }
-      __syncthreads();
-      if (s_piv[par][NW] == 0) {
+      int win[NW];
+      if (!grid_exchange<TIE, NW, W>(a, s_cand[par], s_xw[par], gl, e, win)) {
         if (tid == 0) atomicOr(a.status, kStatusTimeout);
         return;
       }
-      pidx = (uint32_t)s_piv[par][1];
+      pidx = (uint32_t)win[1];
 #pragma unroll
-      for (int j = 0; j < D; ++j) pv[j] = __int_as_float(s_piv[par][2 + j]);
+      for (int j = 0; j < D; ++j) pv[j] = __int_as_float(win[2 + j]);
     }
-    if (g == 0 && tid == 0) a.out[s] = pidx;
+    // The last wave records the pick: it issues no global loads in this
+    // loop, so the store's completion is never waited for.
+    if (g == 0 && tid == B - 64) a.out[s] = pidx;
   }
 }
 
@@ -216,7 +239,7 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
   constexpr int DC = (D > 0) ? D : 1;       // coordinates carried in registers
   constexpr int NW = (D > 0) ? 2 + D : 2;   // words exchanged per candidate
   __shared__ int s_cand[2][NW][W];
-  __shared__ int s_piv[2][NW + 1];
+  __shared__ int s_xw[2][NW + 1][W];
   extern __shared__ float s_pv_dyn[];       // D == 0: pivot coordinates (a.dim)
 
   const int tid = threadIdx.x;
@@ -243,7 +266,7 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
     for (int j = 0; j < DC; ++j) pv[j] = a.pts[(size_t)j * a.ld + pidx];
   } else {
     for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)j * a.ld + pidx];
-    __syncthreads();
+    lds_barrier();
   }
   if (g == 0 && tid == 0) a.out[0] = pidx;
 
@@ -294,11 +317,11 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
 #pragma unroll
         for (int j = 0; j < DC; ++j) pv[j] = a.pts[(size_t)j * a.ld + pidx];
       } else {
-        __syncthreads();  // every lane is done reading the old pivot
+        lds_barrier();  // every lane is done reading the old pivot
         for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)j * a.ld + pidx];
-        __syncthreads();
+        lds_barrier();
       }
-      if (g == 0 && tid == 0) a.out[s] = pidx;
+      if (g == 0 && tid == B - 64) a.out[s] = pidx;
       continue;
     }
     ++e;
@@ -318,7 +341,7 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
         for (int j = 0; j < DC; ++j) s_cand[par][2 + j][wave] = __float_as_int(wc[j]);
       }
     }
-    __syncthreads();
+    lds_barrier();
 
     const int v2 = (lane < W) ? s_cand[par][0][lane] : INT_MIN;
     const uint32_t i2 = (lane < W) ? (uint32_t)s_cand[par][1][lane] : 0u;
@@ -332,23 +355,15 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
         for (int j = 0; j < DC; ++j) pv[j] = __int_as_float(s_cand[par][2 + j][gl]);
       }
     } else {
-      if (wave == 0) {
-        const int mine = (lane < NW) ? s_cand[par][lane][gl] : 0;
-        int win;
-        const bool ok = exchange_wave0<TIE, NW>(a.xchg, G, g, e, mine, win,
-                                                now_ticks() + a.spin_ticks);
-        if (lane < NW) s_piv[par][lane] = win;
-        if (lane == 0) s_piv[par][NW] = ok ? 1 : 0;
-      }
-      __syncthreads();
-      if (s_piv[par][NW] == 0) {
+      int win[NW];
+      if (!grid_exchange<TIE, NW, W>(a, s_cand[par], s_xw[par], gl, e, win)) {
         if (tid == 0) atomicOr(a.status, kStatusTimeout);
         return;
       }
-      pidx = (uint32_t)s_piv[par][1];
+      pidx = (uint32_t)win[1];
       if (D > 0) {
 #pragma unroll
-        for (int j = 0; j < DC; ++j) pv[j] = __int_as_float(s_piv[par][2 + j]);
+        for (int j = 0; j < DC; ++j) pv[j] = __int_as_float(win[2 + j]);
       }
     }
     if (D == 0) {
@@ -356,9 +371,9 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
       // Every lane finished this pass's reads of s_pv_dyn before the
       // candidate barrier above, so the overwrite is safe.
       for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)j * a.ld + pidx];
-      __syncthreads();
+      lds_barrier();
     }
-    if (g == 0 && tid == 0) a.out[s] = pidx;
+    if (g == 0 && tid == B - 64) a.out[s] = pidx;
   }
 }
 

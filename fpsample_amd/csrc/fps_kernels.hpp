@@ -25,7 +25,14 @@ struct FpsParams {
   uint32_t chunk;         // streaming kernels: points per workgroup
   float init;             // initial min-distance
   uint64_t spin_ticks;    // realtime ticks an exchange may wait before giving up
+  uint32_t replicas;      // exchange replicas (1..8); each candidate is written to all
+  uint32_t backoff;       // s_sleep rounds between unanswered polls (0 = spin)
 };
+
+// Exchange-area words the host must provide for a grid of G workgroups.
+inline size_t xchg_words(uint32_t replicas) {
+  return (size_t)2 * replicas * 16 /*kSlotWords*/ * 512 /*kPageWords*/;
+}
 
 struct KernelEntry {
   int dim;    // 0 = any dimension

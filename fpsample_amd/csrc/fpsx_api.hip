@@ -30,6 +30,7 @@
 namespace fpsx {
 
 static thread_local std::string t_err;
+constexpr int kDefaultReplicas = 8;
 
 int fail(int code, const std::string &msg) {
   t_err = msg;
@@ -89,6 +90,7 @@ int get_ctx(Ctx **out) {
   double secs = sec ? std::atof(sec) : 2.0;
   if (!(secs > 0)) secs = 2.0;
   c->spin_ticks = (uint64_t)((double)khz * 1000.0 * secs);
+
   HIP_TRY(hipHostMalloc((void **)&c->host_status, sizeof(uint32_t), hipHostMallocDefault));
   *out = c.get();
   g_ctx[dev] = std::move(c);
@@ -147,6 +149,8 @@ static int plan_stream(Ctx &c, size_t n, size_t dim, int tie, int B, size_t want
   if (gmax == 0) return fail(FPSX_EHIP, "streaming kernel cannot be resident on this device");
   size_t g = want_g ? want_g : (n + B - 1) / B;
   if (g > gmax) g = gmax;
+  if (g > (size_t)B) g = (size_t)B;  // exchange polls one workgroup per lane
+  if (g > (size_t)kPageWords) g = kPageWords;
   if (g < 1) g = 1;
   size_t chunk = (n + g - 1) / g;
   g = (n + chunk - 1) / chunk;  // no empty workgroup
@@ -159,6 +163,8 @@ static int plan_resident_shape(Ctx &c, size_t n, const KernelEntry *k, Plan &p) 
   const size_t per = (size_t)k->block * k->ppt;
   const size_t g = (n + per - 1) / per;
   if (g > 1 && g > (size_t)c.max_blocks(k, 0)) return -1;
+  if (g > (size_t)k->block) return -1;  // exchange polls one workgroup per lane
+  if (g > (size_t)kPageWords) return -1;
   p.kind = PlanKind::Resident;
   p.k = k;
   p.block = (uint32_t)k->block;
@@ -215,7 +221,16 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
 int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t ld, size_t n, size_t dim,
                size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out, float init,
                hipStream_t st) {
-  const size_t xwords = (size_t)2 * kSlotWords * p.grid;
+  // Exchange tuning knobs (read per launch so a tuner can vary them).
+  const char *rep_env = std::getenv("FPSX_XCHG_REPLICAS");
+  const char *bo_env = std::getenv("FPSX_XCHG_BACKOFF");
+  c.replicas = rep_env ? std::atoi(rep_env) : kDefaultReplicas;
+  if (c.replicas < 1) c.replicas = 1;
+  if (c.replicas > kMaxReplicas) c.replicas = kMaxReplicas;
+  c.backoff = bo_env ? std::atoi(bo_env) : 0;
+  if (c.backoff < 0) c.backoff = 0;
+  const uint32_t reps = (uint32_t)c.replicas;
+  const size_t xwords = xchg_words(reps);
   HIP_TRY(c.xchg.reserve(xwords * sizeof(uint64_t)));
   HIP_TRY(c.status.reserve(sizeof(uint64_t)));
   launch_fill_u64((uint64_t *)c.xchg.p, xwords, 0, st);
@@ -234,6 +249,8 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t ld, size_t n, s
   a.status = (uint32_t *)c.status.p;
   a.init = init;
   a.spin_ticks = c.spin_ticks;
+  a.replicas = reps;
+  a.backoff = (uint32_t)c.backoff;
   a.chunk = p.chunk;
   if (p.kind == PlanKind::Stream) {
     HIP_TRY(c.dmin.reserve(n * sizeof(float)));

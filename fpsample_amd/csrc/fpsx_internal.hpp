@@ -30,6 +30,8 @@ struct Ctx {
   hipStream_t stream = nullptr;
   int cus = 0;
   uint64_t spin_ticks = 0;
+  int replicas = 8;  // exchange replicas (FPSX_XCHG_REPLICAS)
+  int backoff = 0;   // poll backoff rounds (FPSX_XCHG_BACKOFF)
   uint32_t *host_status = nullptr;  // pinned
   std::mutex mu;
   DevBuf pts, rows, starts, out, xchg, status, dmin;
