@@ -109,6 +109,30 @@ __device__ __forceinline__ bool grid_exchange(const FpsParams &a, int (*cand)[W]
   return ok_all;
 }
 
+// Records pick s: lane 0 of the calling wave writes the index, lanes j < D the
+// coordinates when the caller asked for them (the fused `pc[idx]` gather).
+template <int D>
+__device__ __forceinline__ void record_pick(const FpsParams &a, uint32_t s, uint32_t pidx,
+                                            const float (&pv)[D]) {
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) a.out[s] = pidx;
+  if (a.out_pts != nullptr && lane < D) {
+    float v = pv[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j) v = (lane == j) ? pv[j] : v;
+    a.out_pts[(size_t)s * D + lane] = v;
+  }
+}
+
+// Same for the any-dimension kernel, coordinates read from LDS.
+__device__ __forceinline__ void record_pick_dyn(const FpsParams &a, uint32_t s, uint32_t pidx,
+                                                const float *pv, uint32_t dim) {
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) a.out[s] = pidx;
+  if (a.out_pts != nullptr)
+    for (uint32_t j = lane; j < dim; j += 64) a.out_pts[(size_t)s * dim + j] = pv[j];
+}
+
 // Coordinates of a forced pick (multi-start prefix).  An index outside the
 // cloud is reported through the status word and replaced by 0, so no access
 // ever leaves the arrays.
@@ -120,7 +144,7 @@ __device__ __forceinline__ uint32_t load_forced(const FpsParams &a, uint32_t s, 
     idx = 0;
   }
 #pragma unroll
-  for (int j = 0; j < D; ++j) pv[j] = a.pts[(size_t)j * a.ld + idx];
+  for (int j = 0; j < D; ++j) pv[j] = a.pts[(size_t)idx * a.sp + (size_t)j * a.ld];
   return (uint32_t)idx;
 }
 
@@ -150,13 +174,13 @@ __global__ void __launch_bounds__(B) fps_resident(FpsParams a) {
     const uint32_t i = base + (uint32_t)(k * B + tid);
     const bool live = i < a.n;
 #pragma unroll
-    for (int j = 0; j < D; ++j) c[k][j] = live ? a.pts[(size_t)j * a.ld + i] : 0.0f;
+    for (int j = 0; j < D; ++j) c[k][j] = live ? a.pts[(size_t)i * a.sp + (size_t)j * a.ld] : 0.0f;
     dm[k] = live ? a.init : -1.0f;
   }
 
   float pv[D];
   uint32_t pidx = load_forced<D>(a, 0, pv);
-  if (g == 0 && tid == 0) a.out[0] = pidx;
+  if (g == 0 && tid >= B - 64) record_pick<D>(a, 0, pidx, pv);
 
   uint32_t e = 0;
   for (uint32_t s = 1; s < a.m; ++s) {
@@ -173,7 +197,7 @@ __global__ void __launch_bounds__(B) fps_resident(FpsParams a) {
     }
     if (s < a.n_starts) {
       pidx = load_forced<D>(a, s, pv);
-      if (g == 0 && tid == B - 64) a.out[s] = pidx;
+      if (g == 0 && tid >= B - 64) record_pick<D>(a, s, pidx, pv);
       continue;
     }
     ++e;
@@ -223,7 +247,7 @@ __global__ void __launch_bounds__(B) fps_resident(FpsParams a) {
     }
     // The last wave records the pick: it issues no global loads in this
     // loop, so the store's completion is never waited for.
-    if (g == 0 && tid == B - 64) a.out[s] = pidx;
+    if (g == 0 && tid >= B - 64) record_pick<D>(a, s, pidx, pv);
   }
 }
 
@@ -263,12 +287,13 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
   }
   if (D > 0) {
 #pragma unroll
-    for (int j = 0; j < DC; ++j) pv[j] = a.pts[(size_t)j * a.ld + pidx];
+    for (int j = 0; j < DC; ++j) pv[j] = a.pts[(size_t)pidx * a.sp + (size_t)j * a.ld];
+    if (g == 0 && tid >= B - 64) record_pick<DC>(a, 0, pidx, pv);
   } else {
-    for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)j * a.ld + pidx];
+    for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)pidx * a.sp + (size_t)j * a.ld];
     lds_barrier();
+    if (g == 0 && tid >= B - 64) record_pick_dyn(a, 0, pidx, s_pv_dyn, dim);
   }
-  if (g == 0 && tid == 0) a.out[0] = pidx;
 
   uint32_t e = 0;
   for (uint32_t s = 1; s < a.m; ++s) {
@@ -282,13 +307,14 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
       float p[DC];
       if (D > 0) {
 #pragma unroll
-        for (int j = 0; j < DC; ++j) p[j] = a.pts[(size_t)j * a.ld + i];
+        for (int j = 0; j < DC; ++j) p[j] = a.pts[(size_t)i * a.sp + (size_t)j * a.ld];
         d = sqdist<DC>(p, pv);
       } else {
-        float df = a.pts[i] - s_pv_dyn[0];
+        const float *pi = a.pts + (size_t)i * a.sp;
+        float df = pi[0] - s_pv_dyn[0];
         d = df * df;
         for (uint32_t j = 1; j < dim; ++j) {
-          float dj = a.pts[(size_t)j * a.ld + i] - s_pv_dyn[j];
+          float dj = pi[(size_t)j * a.ld] - s_pv_dyn[j];
           d = d + dj * dj;
         }
       }
@@ -315,13 +341,16 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
       pidx = (uint32_t)idx;
       if (D > 0) {
 #pragma unroll
-        for (int j = 0; j < DC; ++j) pv[j] = a.pts[(size_t)j * a.ld + pidx];
+        for (int j = 0; j < DC; ++j) pv[j] = a.pts[(size_t)pidx * a.sp + (size_t)j * a.ld];
       } else {
         lds_barrier();  // every lane is done reading the old pivot
-        for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)j * a.ld + pidx];
+        for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)pidx * a.sp + (size_t)j * a.ld];
         lds_barrier();
       }
-      if (g == 0 && tid == B - 64) a.out[s] = pidx;
+      if (g == 0 && tid >= B - 64) {
+        if (D > 0) record_pick<DC>(a, s, pidx, pv);
+        else record_pick_dyn(a, s, pidx, s_pv_dyn, dim);
+      }
       continue;
     }
     ++e;
@@ -370,10 +399,13 @@ __global__ void __launch_bounds__(B) fps_stream(FpsParams a) {
       // Re-read the pivot's coordinates (uniform index, every lane agrees).
       // Every lane finished this pass's reads of s_pv_dyn before the
       // candidate barrier above, so the overwrite is safe.
-      for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)j * a.ld + pidx];
+      for (uint32_t j = tid; j < dim; j += B) s_pv_dyn[j] = a.pts[(size_t)pidx * a.sp + (size_t)j * a.ld];
       lds_barrier();
     }
-    if (g == 0 && tid == B - 64) a.out[s] = pidx;
+    if (g == 0 && tid >= B - 64) {
+      if (D > 0) record_pick<DC>(a, s, pidx, pv);
+      else record_pick_dyn(a, s, pidx, s_pv_dyn, dim);
+    }
   }
 }
 
@@ -390,14 +422,14 @@ __global__ void fill_f32(float *p, size_t n, float v) {
     p[i] = v;
 }
 
-// Row-major (n x dim) -> SoA (dim arrays of stride ld) transpose.
-__global__ void rows_to_soa(const float *__restrict__ rows, float *__restrict__ soa, uint32_t n,
-                            uint32_t dim, uint64_t ld) {
+// Strided (point stride sp, coordinate stride sd) -> SoA (stride ld) copy.
+__global__ void strided_to_soa(const float *__restrict__ src, uint64_t sp, uint64_t sd,
+                               float *__restrict__ soa, uint32_t n, uint32_t dim, uint64_t ld) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)n * dim;
        t += (size_t)gridDim.x * blockDim.x) {
-    const size_t i = t / dim;
-    const size_t j = t - i * dim;
-    soa[j * ld + i] = rows[t];
+    const size_t j = t / n;
+    const size_t i = t - j * n;
+    soa[j * ld + i] = src[i * sp + j * sd];
   }
 }
 
@@ -451,12 +483,12 @@ void launch_fill_f32(float *p, size_t n, float v, hipStream_t st) {
   unsigned blocks = (unsigned)min((size_t)2048, (n + 255) / 256);
   hipLaunchKernelGGL(fill_f32, dim3(blocks), dim3(256), 0, st, p, n, v);
 }
-void launch_rows_to_soa(const float *rows, float *soa, uint32_t n, uint32_t dim, uint64_t ld,
-                        hipStream_t st) {
+void launch_strided_to_soa(const float *src, uint64_t sp, uint64_t sd, float *soa, uint32_t n,
+                           uint32_t dim, uint64_t ld, hipStream_t st) {
   size_t total = (size_t)n * dim;
   if (total == 0) return;
   unsigned blocks = (unsigned)min((size_t)4096, (total + 255) / 256);
-  hipLaunchKernelGGL(rows_to_soa, dim3(blocks), dim3(256), 0, st, rows, soa, n, dim, ld);
+  hipLaunchKernelGGL(strided_to_soa, dim3(blocks), dim3(256), 0, st, src, sp, sd, soa, n, dim, ld);
 }
 
 }  // namespace fpsx

@@ -11,14 +11,16 @@ constexpr uint32_t kStatusBadStart = 2u;  // a start index >= n (replaced by 0)
 
 // Kernel arguments (passed by value).
 struct FpsParams {
-  const float *pts;       // SoA: coordinate j of point i at pts[j * ld + i]
-  uint64_t ld;            // >= n
+  const float *pts;       // coordinate j of point i at pts[i * sp + j * ld]
+  uint64_t ld;            // stride between coordinates (SoA: >= n; row-major: 1)
+  uint64_t sp;            // stride between points (SoA: 1; row-major: dim)
   uint32_t n;             // number of points (< 2^31)
   uint32_t m;             // number of samples (1..n)
   const uint64_t *starts; // n_starts forced picks (device)
   uint32_t n_starts;      // >= 1
   uint32_t dim;           // runtime dimension (generic streaming kernel)
   uint64_t *out;          // m indices (device)
+  float *out_pts;         // optional m x dim row-major coordinates of the picks (or null)
   uint64_t *xchg;         // exchange area, 2 * kSlotWords * G words, zeroed before launch
   uint32_t *status;       // status word, zeroed before launch
   float *dmin;            // streaming kernels: n min-distances, filled with init
@@ -47,7 +49,9 @@ const KernelEntry *stream_table(size_t *count);
 
 void launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t st);
 void launch_fill_f32(float *p, size_t n, float v, hipStream_t st);
-void launch_rows_to_soa(const float *rows, float *soa, uint32_t n, uint32_t dim, uint64_t ld,
-                        hipStream_t st);
+// Copy a strided point array (coordinate j of point i at src[i*sp + j*sd])
+// into SoA layout (soa[j*ld + i]).
+void launch_strided_to_soa(const float *src, uint64_t sp, uint64_t sd, float *soa, uint32_t n,
+                           uint32_t dim, uint64_t ld, hipStream_t st);
 
 }  // namespace fpsx

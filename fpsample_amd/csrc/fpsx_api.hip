@@ -218,9 +218,17 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
 // ---------------------------------------------------------------------------
 // Launch on device-resident data.  Caller holds c.mu.
 // ---------------------------------------------------------------------------
-int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t ld, size_t n, size_t dim,
-               size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out, float init,
-               hipStream_t st) {
+int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, size_t n,
+               size_t dim, size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out,
+               float *d_out_pts, float init, hipStream_t st) {
+  if (p.kind == PlanKind::Stream && sp != 1) {
+    // The streaming kernel re-reads every point on every pass: give it SoA.
+    HIP_TRY(c.pts.reserve(n * dim * sizeof(float)));
+    launch_strided_to_soa(d_pts, sp, ld, (float *)c.pts.p, (uint32_t)n, (uint32_t)dim, n, st);
+    d_pts = (const float *)c.pts.p;
+    sp = 1;
+    ld = n;
+  }
   // Exchange tuning knobs (read per launch so a tuner can vary them).
   const char *rep_env = std::getenv("FPSX_XCHG_REPLICAS");
   const char *bo_env = std::getenv("FPSX_XCHG_BACKOFF");
@@ -239,6 +247,8 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t ld, size_t n, s
   std::memset(&a, 0, sizeof(a));
   a.pts = d_pts;
   a.ld = ld;
+  a.sp = sp;
+  a.out_pts = d_out_pts;
   a.n = (uint32_t)n;
   a.m = (uint32_t)m;
   a.starts = d_starts;
@@ -269,9 +279,18 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t ld, size_t n, s
     ev = &c.events[c.events_used++];
     HIP_TRY(hipEventRecord(ev->first, st));
   }
-  if (p.grid > 1) {
+  // The planner bounds the grid by the co-resident capacity and every
+  // exchange wait is bounded, so a plain launch is used: were some workgroup
+  // not resident, the kernel would report FPSX_ETIMEOUT rather than hang.
+  // FPSX_COOPERATIVE=1 asks the runtime to validate co-residency instead
+  // (same speed; some profilers crash on cooperative dispatches).
+  const char *coop_env = std::getenv("FPSX_COOPERATIVE");
+  const bool coop = coop_env && coop_env[0] == '1';
+  if (p.grid > 1 && coop) {
     HIP_TRY(hipLaunchCooperativeKernel(p.k->fn, dim3(p.grid), dim3(p.block), args,
                                        (unsigned)p.shmem, st));
+  } else if (p.grid > 1) {
+    HIP_TRY(hipLaunchKernel(p.k->fn, dim3(p.grid), dim3(p.block), args, p.shmem, st));
   } else {
     HIP_TRY(hipLaunchKernel(p.k->fn, dim3(1), dim3(p.block), args, p.shmem, st));
   }
@@ -307,13 +326,13 @@ int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
   } else {
     HIP_TRY(c.rows.reserve(bytes));
     HIP_TRY(hipMemcpyAsync(c.rows.p, points, bytes, hipMemcpyHostToDevice, st));
-    launch_rows_to_soa((const float *)c.rows.p, (float *)c.pts.p, (uint32_t)n, (uint32_t)dim, n,
-                       st);
+    launch_strided_to_soa((const float *)c.rows.p, dim, 1, (float *)c.pts.p, (uint32_t)n,
+                          (uint32_t)dim, n, st);
   }
   HIP_TRY(hipMemcpyAsync(c.starts.p, starts, n_starts * sizeof(uint64_t), hipMemcpyHostToDevice,
                          st));
-  rc = launch_fps(c, p, (const float *)c.pts.p, n, n, dim, m, (const uint64_t *)c.starts.p,
-                  n_starts, (uint64_t *)c.out.p, init, st);
+  rc = launch_fps(c, p, (const float *)c.pts.p, 1, n, n, dim, m, (const uint64_t *)c.starts.p,
+                  n_starts, (uint64_t *)c.out.p, nullptr, init, st);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(out, c.out.p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(c.host_status, c.status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -357,14 +376,17 @@ FPSX_EXPORT int fpsx_fps_sampling(const float *points, size_t n_points, size_t d
                   kTieHighest, INFINITY);
 }
 
-FPSX_EXPORT int fpsx_fps_sampling_device(const float *d_points, size_t ld, size_t n_points,
-                                         size_t dim, size_t n_samples, const uint64_t *d_start,
-                                         size_t n_start, uint64_t *d_out, int tie, float init,
-                                         void *stream) {
+FPSX_EXPORT int fpsx_fps_sampling_device_strided(const float *d_points, size_t stride_point,
+                                                 size_t stride_dim, size_t n_points, size_t dim,
+                                                 size_t n_samples, const uint64_t *d_start,
+                                                 size_t n_start, uint64_t *d_out,
+                                                 float *d_out_points, int tie, float init,
+                                                 void *stream) {
   t_err.clear();
   if (dim == 0) return fail(FPSX_EBADDIM, "dim must be >= 1");
   if (!d_points || !d_start || !d_out) return fail(FPSX_EINVAL, "null pointer argument");
-  if (ld < n_points) return fail(FPSX_EINVAL, "ld must be >= n_points");
+  if (stride_point == 0 || (dim > 1 && stride_dim == 0))
+    return fail(FPSX_EINVAL, "zero stride");
   if (n_points == 0 || n_samples == 0) return fail(FPSX_EINVAL, "empty cloud or no samples");
   if (n_start == 0 || n_start > n_samples)
     return fail(FPSX_EINVAL, "the number of start indices must be in [1, n_samples]");
@@ -380,8 +402,20 @@ FPSX_EXPORT int fpsx_fps_sampling_device(const float *d_points, size_t ld, size_
   rc = plan(c, n_points, dim, tie, p);
   if (rc) return rc;
   hipStream_t st = stream ? (hipStream_t)stream : c.stream;
-  return launch_fps(c, p, d_points, ld, n_points, dim, n_samples, d_start, n_start, d_out, init,
-                    st);
+  return launch_fps(c, p, d_points, stride_point, stride_dim, n_points, dim, n_samples, d_start,
+                    n_start, d_out, d_out_points, init, st);
+}
+
+FPSX_EXPORT int fpsx_fps_sampling_device(const float *d_points, size_t ld, size_t n_points,
+                                         size_t dim, size_t n_samples, const uint64_t *d_start,
+                                         size_t n_start, uint64_t *d_out, int tie, float init,
+                                         void *stream) {
+  if (ld < n_points) {
+    t_err.clear();
+    return fail(FPSX_EINVAL, "ld must be >= n_points");
+  }
+  return fpsx_fps_sampling_device_strided(d_points, 1, ld, n_points, dim, n_samples, d_start,
+                                          n_start, d_out, nullptr, tie, init, stream);
 }
 
 FPSX_EXPORT int fpsx_device_status(void *stream) {

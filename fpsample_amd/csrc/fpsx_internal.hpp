@@ -57,9 +57,11 @@ struct Plan {
 int fail(int code, const std::string &msg);
 int get_ctx(Ctx **out);
 int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p);
-int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t ld, size_t n, size_t dim,
-               size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out, float init,
-               hipStream_t st);
+// Enqueue one FPS run on device data: coordinate j of point i at
+// d_pts[i*sp + j*ld].  d_out_pts (may be null) receives the picks' coordinates.
+int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, size_t n,
+               size_t dim, size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out,
+               float *d_out_pts, float init, hipStream_t st);
 int status_code(uint32_t st);
 int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
              const uint64_t *starts, size_t n_starts, uint64_t *out, int tie, float init);

@@ -59,6 +59,9 @@ def lib() -> ctypes.CDLL:
         L.fpsx_fps_sampling_device.argtypes = [vp, sz, sz, sz, sz, vp, sz, vp, ctypes.c_int,
                                                ctypes.c_float, vp]
         L.fpsx_fps_sampling_device.restype = ctypes.c_int
+        L.fpsx_fps_sampling_device_strided.argtypes = [vp, sz, sz, sz, sz, sz, vp, sz, vp, vp,
+                                                       ctypes.c_int, ctypes.c_float, vp]
+        L.fpsx_fps_sampling_device_strided.restype = ctypes.c_int
         L.fpsx_device_status.argtypes = [vp]
         L.fpsx_timing.argtypes = [ctypes.c_int]
         L.fpsx_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
@@ -96,6 +99,11 @@ class DeviceArray:
         _check(hip().hipMemcpy(a.ctypes.data, self.ptr, self.nbytes, _D2H), "hipMemcpy D2H")
         return a
 
+    @property
+    def __cuda_array_interface__(self):
+        return {"shape": self.shape, "typestr": self.dtype.str, "data": (self.ptr, False),
+                "strides": None, "version": 3}
+
     def free(self) -> None:
         if self.ptr:
             hip().hipFree(self.ptr)
@@ -124,6 +132,66 @@ def fps_sampling_device(points, n_points: int, dim: int, n_samples: int, starts,
                                         n_points, dim, n_samples, _addr(starts), n_starts,
                                         _addr(out), tie, init, stream)
     _raise(rc, "fpsx_fps_sampling_device")
+
+
+def fps_sampling_device_strided(points, stride_point: int, stride_dim: int, n_points: int,
+                                dim: int, n_samples: int, starts, n_starts: int, out,
+                                out_points=None, tie: int = TIE_HIGHEST,
+                                init: float = float("inf"), stream: Optional[int] = None) -> None:
+    """Enqueue FPS on device data of any layout (strides in elements); optionally
+    write the picks' coordinates (n_samples x dim float32) to out_points."""
+    rc = lib().fpsx_fps_sampling_device_strided(
+        _addr(points), stride_point, stride_dim, n_points, dim, n_samples, _addr(starts),
+        n_starts, _addr(out), _addr(out_points) if out_points is not None else None, tie, init,
+        stream)
+    _raise(rc, "fpsx_fps_sampling_device_strided")
+
+
+def _tensor_view(t):
+    """(address, shape, strides in elements) of a 2-D float32 device array:
+    a torch tensor on the GPU, or anything exposing __cuda_array_interface__."""
+    if hasattr(t, "data_ptr") and hasattr(t, "stride"):
+        if str(getattr(t, "dtype", "")) not in ("torch.float32", "float32"):
+            raise TypeError("points must be float32")
+        return int(t.data_ptr()), tuple(t.shape), tuple(t.stride())
+    cai = getattr(t, "__cuda_array_interface__", None)
+    if cai is None:
+        raise TypeError("expected a device tensor (torch) or an object with __cuda_array_interface__")
+    if np.dtype(cai["typestr"]) != np.float32:
+        raise TypeError("points must be float32")
+    shape = tuple(cai["shape"])
+    strides = cai.get("strides")
+    if strides is None:
+        strides = (shape[1] * 4, 4)
+    return int(cai["data"][0]), shape, tuple(s // 4 for s in strides)
+
+
+def fps_sampling_tensor(points, n_samples: int, start_idx: Union[int, Sequence[int]] = 0,
+                        return_points: bool = False, stream: Optional[int] = None):
+    """FPS on a device-resident (n_points, dim) float32 tensor, used in place.
+
+    Returns (indices, coordinates-or-None) as DeviceArrays: indices uint64
+    (n_samples,), coordinates float32 (n_samples, dim) when return_points.
+    The work is enqueued on `stream`; call synchronize() (or read the arrays,
+    which synchronises) before use.
+    """
+    addr, shape, strides = _tensor_view(points)
+    if len(shape) != 2:
+        raise ValueError("points must be a 2-D (n_points, dim) tensor")
+    n, d = shape
+    starts = np.atleast_1d(np.asarray(start_idx, dtype=np.uint64))
+    if n_samples < 1 or n_samples > n:
+        raise ValueError("n_samples must be in [1, n_points]")
+    if starts.size < 1 or starts.size > n_samples or np.any(starts >= n):
+        raise ValueError("start indices must be < n_points, 1 <= count <= n_samples")
+    d_start = DeviceArray.from_numpy(starts)
+    out = DeviceArray((n_samples,), np.uint64)
+    out_pts = DeviceArray((n_samples, d), np.float32) if return_points else None
+    fps_sampling_device_strided(addr, strides[0], strides[1], n, d, n_samples, d_start,
+                                starts.size, out, out_pts, stream=stream)
+    synchronize()
+    d_start.free()
+    return out, out_pts
 
 
 def device_status(stream: Optional[int] = None) -> None:

@@ -110,6 +110,19 @@ int fpsx_fps_sampling_device(const float *d_points, size_t ld, size_t n_points, 
                              uint64_t *d_out, int tie, float init, void *stream);
 
 /*
+ * General device form: coordinate j of point i at d_points[i*stride_point +
+ * j*stride_dim] (SoA: 1, ld; a row-major n x dim tensor: dim, 1), so device
+ * tensors of either layout are used in place.  d_out_points (may be NULL)
+ * receives the picks' coordinates as an n_samples x dim row-major float32
+ * array, written by the kernel as it picks (the `pc[idx]` gather fused in).
+ * Streaming plans copy a non-SoA input into the library's SoA scratch first.
+ */
+int fpsx_fps_sampling_device_strided(const float *d_points, size_t stride_point, size_t stride_dim,
+                                     size_t n_points, size_t dim, size_t n_samples,
+                                     const uint64_t *d_start, size_t n_start, uint64_t *d_out,
+                                     float *d_out_points, int tie, float init, void *stream);
+
+/*
  * Bucket FPS entry with the exact signature and return codes of the
  * reference's C API bucket_fps_kdline (src/wrapper.hpp:118-132).
  * Same picks as the reference: the KD-line tree of height `height` is built on

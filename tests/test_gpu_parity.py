@@ -199,3 +199,28 @@ def test_lattice_ties_streaming_size(gpu):
     assert len(np.unique(out)) == m
     resumed = run_fps(gpu, pc, m, out[:3000])
     np.testing.assert_array_equal(resumed, out)
+
+
+def test_device_strided_rows_and_fused_gather(gpu, plan_env):
+    from fpsample_amd import device as D
+
+    rng = np.random.default_rng(21)
+    for d, plan in ((3, None), (3, "stream:1024:4"), (5, None), (11, None)):
+        plan_env(plan)
+        pc = rng.standard_normal((6000, d)).astype(np.float32)
+        rows = D.DeviceArray.from_numpy(pc)               # row-major: strides (d, 1)
+        want = O.fps(pc, 400, 17)
+        idx, pts = D.fps_sampling_tensor(rows, 400, 17, return_points=True)
+        got = idx.numpy()
+        np.testing.assert_array_equal(got, want, err_msg=f"d={d} plan={plan}")
+        np.testing.assert_array_equal(pts.numpy(), pc[want.astype(np.int64)])
+        soa = D.DeviceArray.from_numpy(np.ascontiguousarray(pc.T))
+        ds = D.DeviceArray.from_numpy(np.array([17, 3], np.uint64))
+        out = D.DeviceArray((400,), np.uint64)
+        opts = D.DeviceArray((400, d), np.float32)
+        D.fps_sampling_device_strided(soa, 1, 6000, 6000, d, 400, ds, 2, out, opts)
+        D.synchronize()
+        D.device_status()
+        want2 = O.fps(pc, 400, [17, 3])
+        np.testing.assert_array_equal(out.numpy(), want2)
+        np.testing.assert_array_equal(opts.numpy(), pc[want2.astype(np.int64)])
