@@ -4,7 +4,8 @@
 A step is one vanilla FPS run (fps_sampling semantics, src/lib.cpp:188-246)
 over one synthetic cloud already resident in HBM: one call of
 fpsx_fps_sampling_device, i.e. the exchange-area clear and the single
-persistent FPS launch that makes all M-1 passes.
+persistent FPS launch that makes all M-1 passes.  Before the warmup the
+library's autotuner (fpsx_autotune, untimed) picks the kernel shape.
 
 Default workload (the metric's configuration): one cloud of N = 1,048,576
 points, D = 3 float32, a mixture of 64 isotropic Gaussian clusters (sigma
@@ -134,6 +135,7 @@ def main() -> int:
     ap.add_argument("--workload", default="gauss1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--autotune", default="on", choices=["on", "off"])
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -152,6 +154,8 @@ def main() -> int:
     d_pts = D.DeviceArray.from_numpy(soa)
     d_start = D.DeviceArray.from_numpy(np.zeros(1, np.uint64))
     d_out = D.DeviceArray((m,), np.uint64)
+    if args.autotune == "on":
+        D.autotune(d_pts, 1, n, n, 3)  # untimed: picks the kernel shape for this cloud
     plan = D.plan_info(n, 3)
 
     def step():

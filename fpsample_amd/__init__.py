@@ -23,11 +23,12 @@ try:
         __version__,
         _bucket_fps_kdline_sampling,
         _fps_sampling,
+        _fps_sampling_batch,
     )
 except ImportError as exc:  # pragma: no cover - exercised only on a broken install
     raise ImportError(
         "fpsample_amd: the native extension (_fpsample + libfpsx.so) is not built; "
-        "run `python -m fpsample_amd.build`"
+        "run `python fpsample_amd/build.py`"
     ) from exc
 
 
@@ -117,6 +118,45 @@ def bucket_fps_kdline_sampling(
     return _bucket_fps_kdline_sampling(pc, n_samples, h, start_idx)
 
 
+def fps_sampling_batch(
+    pcs: List[np.ndarray],
+    n_samples: Union[int, List[int]],
+    start_idx: Optional[Union[int, List[int]]] = None,
+) -> List[np.ndarray]:
+    """
+    Vanilla FPS over many clouds in one GPU launch (no counterpart in the reference).
+
+    Each cloud's picks equal ``fps_sampling(pc, n, start_idx=s)`` for that cloud
+    alone.  Clouds may differ in size but must share the number of columns.
+
+    Args:
+        pcs (list[np.ndarray]): point clouds of shape (n_i, D).
+        n_samples (int or list[int]): samples per cloud (one value for all, or one each).
+        start_idx (None, int or list[int]): first pick per cloud; None draws one
+            np.random.randint(0, n_i) per cloud, in order.
+    Returns:
+        list[np.ndarray]: uint64 picks per cloud.
+    """
+    assert isinstance(pcs, (list, tuple)) and len(pcs) >= 1, "pcs should be a non-empty list"
+    k = len(pcs)
+    ns = [int(n_samples)] * k if isinstance(n_samples, (int, np.integer)) else [int(v) for v in n_samples]
+    assert len(ns) == k, "n_samples should have one entry per cloud"
+    if start_idx is None:
+        starts = [int(np.random.randint(low=0, high=pc.shape[0])) for pc in pcs]
+    elif isinstance(start_idx, int):
+        starts = [start_idx] * k
+    else:
+        starts = [int(v) for v in start_idx]
+    assert len(starts) == k, "start_idx should have one entry per cloud"
+    for pc, n, s in zip(pcs, ns, starts):
+        assert pc.ndim == 2
+        assert n >= 1, "n_samples should be >= 1"
+        assert pc.shape[0] >= n, "n_pts should be >= n_samples"
+        assert 0 <= s < pc.shape[0], "start_idx should be None or 0 <= start_idx < n_pts"
+    clouds = [np.ascontiguousarray(pc, dtype=np.float32) for pc in pcs]
+    return _fps_sampling_batch(clouds, ns, starts)
+
+
 __doc__ = _ext_doc
 
 __all__ = [
@@ -124,5 +164,6 @@ __all__ = [
     "__version__",
     "fps_sampling",
     "bucket_fps_kdline_sampling",
+    "fps_sampling_batch",
     "get_start_idx",
 ]

@@ -8,7 +8,7 @@ Device code is compiled with -ffp-contract=off (the distance is summed exactly
 as the reference sums it, with no fused multiply-add), with f32 denormals kept,
 and with packed FP32 instructions disabled.
 
-Usage: python -m fpsample_amd.build   (or fpsample_amd.build.build())
+Usage: python fpsample_amd/build.py [-v]
 """
 from __future__ import annotations
 
@@ -36,7 +36,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", CSRC, "-I", INCLUDE,
 ]
 
-SOURCES_HIP = ["fps_kernels.hip", "fpsx_api.hip"]
+SOURCES_HIP = ["fps_kernels.hip", "fps_batch.hip", "fpsx_api.hip", "fpsx_batch.hip"]
 SOURCES_HOST = ["kdline.cpp"]
 
 

@@ -1,0 +1,49 @@
+// fps_batch.hpp -- host-visible description of the batched FPS kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fps_kernels.hpp"
+
+namespace fpsx {
+
+constexpr uint32_t kMaxTeam = 64;                     // workgroups per cloud at most
+constexpr size_t kTeamXchgWords = 2 * 16 * kMaxTeam;  // 2 buffers x kSlotWords x kMaxTeam
+
+// One cloud of a batch (and the passes this launch makes on it).
+struct BatchJob {
+  uint64_t pt_off;     // first point of the cloud in the concatenated SoA arrays
+  uint64_t out_off;    // first pick of the cloud in out[]
+  uint64_t state_off;  // first saved min-distance of the cloud in state[] (chunked runs)
+  uint64_t start;      // pick 0
+  uint32_t n;          // points
+  uint32_t m;          // samples
+  uint32_t s_begin;    // this launch computes picks s_begin .. s_end-1 (0 = fresh start)
+  uint32_t s_end;
+  uint32_t team;       // workgroups working on the cloud (1..kMaxTeam)
+  uint32_t member_base;// claim slot of the team's first member
+  uint32_t share;      // points per member: ceil(n / team) <= B * PPT
+  uint32_t area;       // exchange area of the team (teams only)
+};
+
+struct BatchParams {
+  const float *pts;          // concatenated SoA: coordinate d of point p at pts[d * ld + p]
+  uint64_t ld;
+  const BatchJob *jobs;
+  const uint32_t *member_job;  // claim slot -> job
+  uint32_t total_members;
+  uint32_t *claim;           // claim counter, zeroed before the launch
+  uint32_t *ready;           // per-job team-formation counters, zeroed
+  uint64_t *out;             // concatenated picks
+  float *state;              // saved min-distances (chunked runs; may be null otherwise)
+  uint64_t *xchg;            // team exchange areas (kTeamXchgWords each), zeroed
+  uint32_t *status;          // status word, zeroed
+  float init;                // initial min-distance
+  uint64_t spin_ticks;       // bound on one exchange wait
+  uint64_t team_ticks;       // bound on waiting for a team to form
+};
+
+const KernelEntry *batch_table(size_t *count);
+
+}  // namespace fpsx

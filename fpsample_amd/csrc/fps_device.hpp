@@ -154,4 +154,90 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// ---------------------------------------------------------------------------
+// Cross-workgroup step among G participants (a whole grid, or a team of
+// workgroups working on one cloud), entered by every wave of participant g
+// once the workgroup's candidate is known (word k of it in cand[k][gl],
+// k < NW).  Word k of participant g's candidate for buffer buf, replica r
+// lives at base[((buf * R + r) * kSlotWords + k) * pitch + g] (pitch >= G).
+// Wave 0 publishes to every replica the candidate size allows; waves
+// 0..ceil(G/64)-1 (<= W, guaranteed by the host) each poll 64 slots of replica
+// g mod R, one slot per lane with all NW words loaded at once, reduce them and
+// leave their winner in xw[k][wave] (xw[NW][wave] = arrived in time); after
+// one LDS barrier every wave reduces those winners.  On return win[k] holds
+// word k of the winner in every lane.  Returns false when some candidate did
+// not arrive within spin_ticks.
+// ---------------------------------------------------------------------------
+template <int TIE, int NW, int W>
+__device__ __forceinline__ bool exchange(uint64_t *base, uint32_t R, uint32_t pitch, uint32_t G,
+                                         uint32_t g, uint32_t e, uint32_t backoff,
+                                         uint64_t spin_ticks, int (*cand)[W], int (*xw)[W],
+                                         int gl, int (&win)[NW]) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int P = (int)((G + 63u) >> 6);
+  const int K = min((int)R, 64 / NW);  // replicas this candidate size allows
+  uint64_t *bufs = base + (size_t)(e & 1u) * R * kSlotWords * pitch;
+  if (wave == 0 && lane < K * NW) {
+    const int rep = lane / NW;
+    const int k = lane - rep * NW;
+    xchg_store(bufs + ((size_t)rep * kSlotWords + k) * pitch + g,
+               ((uint64_t)(uint32_t)cand[k][gl] << 32) | e);
+  }
+  uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * pitch;
+  if (wave < P) {
+    const uint32_t q = (uint32_t)(wave * 64 + lane);
+    int cv = INT_MIN;
+    uint32_t ci = 0;
+    int cw[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) cw[k] = 0;
+    bool ok = true;
+    if (q < G) {
+      uint64_t w[NW];
+      const uint64_t deadline = now_ticks() + spin_ticks;
+      for (;;) {
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) w[k] = xchg_load(buf + (size_t)k * pitch + q);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) all = all && ((uint32_t)w[k] == e);
+        if (all) break;
+        if (now_ticks() > deadline) {
+          ok = false;
+          break;
+        }
+        for (uint32_t z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(1);
+      }
+      if (ok) {
+        cv = (int)(w[0] >> 32);
+        ci = (uint32_t)(w[1] >> 32);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) cw[k] = (int)(w[k] >> 32);
+      }
+    }
+    int xv;
+    const int xl = wave_argmax_lane<TIE>(cv, ci, xv);
+    const bool wave_ok = __ballot(!ok) == 0;
+    int word = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const int x = readlane_i(cw[k], xl);
+      if (lane == k) word = x;
+    }
+    if (lane < NW) xw[lane][wave] = word;
+    if (lane == NW) xw[NW][wave] = wave_ok ? 1 : 0;
+  }
+  lds_barrier();
+  bool ok_all = true;
+  for (int r = 0; r < P; ++r) ok_all = ok_all && (xw[NW][r] != 0);
+  const int v3 = (lane < P) ? xw[0][lane] : INT_MIN;
+  const uint32_t i3 = (lane < P) ? (uint32_t)xw[1][lane] : 0u;
+  int gv;
+  const int wl = wave_argmax_lane<TIE>(v3, i3, gv);
+#pragma unroll
+  for (int k = 0; k < NW; ++k) win[k] = xw[k][wl];
+  return ok_all;
+}
+
 }  // namespace fpsx

@@ -27,86 +27,12 @@
 
 namespace fpsx {
 
-// ---------------------------------------------------------------------------
-// Cross-workgroup step, entered by every wave once the workgroup's candidate
-// is known (word k of it in cand[k][gl], k < NW).  Wave 0 publishes the
-// candidate; waves 0..R-1 (R = ceil(G/64), host guarantees R <= W) each poll
-// 64 slots, one slot per lane with all NW words loaded at once, reduce them
-// and leave their winner in xw[k][wave] (xw[NW][wave] = arrived in time);
-// after one LDS barrier every wave reduces the R winners.  On return win[k]
-// holds word k of the grid's winner in every lane.  Returns false when some
-// candidate did not arrive within a.spin_ticks.
-// ---------------------------------------------------------------------------
+// The whole grid as one set of participants (single-cloud kernels).
 template <int TIE, int NW, int W>
 __device__ __forceinline__ bool grid_exchange(const FpsParams &a, int (*cand)[W], int (*xw)[W],
                                               int gl, uint32_t e, int (&win)[NW]) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const uint32_t G = gridDim.x;
-  const uint32_t g = blockIdx.x;
-  const int R = (int)((G + 63u) >> 6);
-  const int K = min((int)a.replicas, 64 / NW);  // replicas this candidate size allows
-  uint64_t *bufs = a.xchg + (size_t)(e & 1u) * a.replicas * kSlotWords * kPageWords;
-  if (wave == 0 && lane < K * NW) {
-    const int rep = lane / NW;
-    const int k = lane - rep * NW;
-    xchg_store(bufs + ((size_t)rep * kSlotWords + k) * kPageWords + g,
-               ((uint64_t)(uint32_t)cand[k][gl] << 32) | e);
-  }
-  uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * kPageWords;
-  if (wave < R) {
-    const uint32_t q = (uint32_t)(wave * 64 + lane);
-    int cv = INT_MIN;
-    uint32_t ci = 0;
-    int cw[NW];
-#pragma unroll
-    for (int k = 0; k < NW; ++k) cw[k] = 0;
-    bool ok = true;
-    if (q < G) {
-      uint64_t w[NW];
-      const uint64_t deadline = now_ticks() + a.spin_ticks;
-      for (;;) {
-        bool all = true;
-#pragma unroll
-        for (int k = 0; k < NW; ++k) w[k] = xchg_load(buf + (size_t)k * kPageWords + q);
-#pragma unroll
-        for (int k = 0; k < NW; ++k) all = all && ((uint32_t)w[k] == e);
-        if (all) break;
-        if (now_ticks() > deadline) {
-          ok = false;
-          break;
-        }
-        for (uint32_t z = 0; z < a.backoff; ++z) __builtin_amdgcn_s_sleep(1);
-      }
-      if (ok) {
-        cv = (int)(w[0] >> 32);
-        ci = (uint32_t)(w[1] >> 32);
-#pragma unroll
-        for (int k = 0; k < NW; ++k) cw[k] = (int)(w[k] >> 32);
-      }
-    }
-    int xv;
-    const int xl = wave_argmax_lane<TIE>(cv, ci, xv);
-    const bool wave_ok = __ballot(!ok) == 0;
-    int word = 0;
-#pragma unroll
-    for (int k = 0; k < NW; ++k) {
-      const int x = readlane_i(cw[k], xl);
-      if (lane == k) word = x;
-    }
-    if (lane < NW) xw[lane][wave] = word;
-    if (lane == NW) xw[NW][wave] = wave_ok ? 1 : 0;
-  }
-  lds_barrier();
-  bool ok_all = true;
-  for (int r = 0; r < R; ++r) ok_all = ok_all && (xw[NW][r] != 0);
-  const int v3 = (lane < R) ? xw[0][lane] : INT_MIN;
-  const uint32_t i3 = (lane < R) ? (uint32_t)xw[1][lane] : 0u;
-  int gv;
-  const int wl = wave_argmax_lane<TIE>(v3, i3, gv);
-#pragma unroll
-  for (int k = 0; k < NW; ++k) win[k] = xw[k][wl];
-  return ok_all;
+  return exchange<TIE, NW, W>(a.xchg, a.replicas, kPageWords, gridDim.x, blockIdx.x, e, a.backoff,
+                              a.spin_ticks, cand, xw, gl, win);
 }
 
 // Records pick s: lane 0 of the calling wave writes the index, lanes j < D the

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -60,6 +61,12 @@ hipError_t DevBuf::reserve(size_t bytes) {
   hipError_t e = hipMalloc(&p, want);
   if (e == hipSuccess) cap = want;
   return e;
+}
+
+void DevBuf::free() {
+  if (p) hipFree(p);
+  p = nullptr;
+  cap = 0;
 }
 
 DevBuf::~DevBuf() {
@@ -174,6 +181,22 @@ static int plan_resident_shape(Ctx &c, size_t n, const KernelEntry *k, Plan &p) 
   return FPSX_OK;
 }
 
+// Per-pass time model fitted to MI355X measurements (microseconds):
+// reduction chain ~0.55, update 12 VALU ops per point over 64 lanes/clock
+// at 2.4 GHz, exchange ~1.15 + 0.009 * G.
+static double model_us(size_t n, size_t g) {
+  const double per_wg = (double)n / (double)g;
+  return 0.55 + per_wg * 12.0 / (64.0 * 2400.0) + (g > 1 ? 1.15 + 0.009 * (double)g : 0.0);
+}
+
+static int plan_from_tuned(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
+  auto it = c.tuned.find(std::make_tuple(n, dim, tie));
+  if (it == c.tuned.end()) return -1;
+  const KernelEntry *k = it->second.first;
+  if (k->ppt > 0) return plan_resident_shape(c, n, k, p) == FPSX_OK ? FPSX_OK : -1;
+  return plan_stream(c, n, dim, tie, k->block, it->second.second, k->dim == 0, p);
+}
+
 int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
   const char *env = std::getenv("FPSX_PLAN");
   if (env && *env) {
@@ -196,6 +219,8 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
     return fail(FPSX_EINVAL, "FPSX_PLAN: bad value " + s);
   }
 
+  if (plan_from_tuned(c, n, dim, tie, p) == FPSX_OK) return FPSX_OK;
+
   // Single workgroup: the smallest resident shape that holds the cloud.
   static const int kSmall[][2] = {{256, 1}, {256, 2}, {256, 4}, {1024, 1}, {1024, 2},
                                   {1024, 4}, {1024, 8}, {1024, 16}};
@@ -206,13 +231,44 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
       if (k && plan_resident_shape(c, n, k, p) == FPSX_OK) return FPSX_OK;
     }
   }
-  // Multi-workgroup resident: grow the per-workgroup share as needed.
-  static const int kMulti[][2] = {{1024, 4}, {1024, 8}, {1024, 16}, {256, 32}};
-  for (auto &sh : kMulti) {
-    const KernelEntry *k = find_resident((int)dim, sh[0], sh[1], tie);
-    if (k && plan_resident_shape(c, n, k, p) == FPSX_OK) return FPSX_OK;
+  // Multi-workgroup resident: the feasible shape the time model prefers.
+  size_t cnt;
+  const KernelEntry *t = resident_table(&cnt);
+  double best = 1e30;
+  Plan bp;
+  for (size_t i = 0; i < cnt; ++i) {
+    if (t[i].dim != (int)dim || t[i].tie != tie) continue;
+    Plan q;
+    if (plan_resident_shape(c, n, &t[i], q) != FPSX_OK) continue;
+    const double us = model_us(n, q.grid) + (t[i].block == 1024 ? 0.0 : 0.05);
+    if (us < best) {
+      best = us;
+      bp = q;
+    }
+  }
+  if (bp.k) {
+    p = bp;
+    return FPSX_OK;
   }
   return plan_stream(c, n, dim, tie, 1024, 0, false, p);
+}
+
+// Candidate plans the tuner times for a cloud: every feasible resident shape
+// of this dimension plus the streaming kernel at two block sizes.
+static std::vector<Plan> candidates(Ctx &c, size_t n, size_t dim, int tie) {
+  std::vector<Plan> out;
+  size_t cnt;
+  const KernelEntry *t = resident_table(&cnt);
+  for (size_t i = 0; i < cnt; ++i) {
+    if (t[i].dim != (int)dim || t[i].tie != tie) continue;
+    Plan q;
+    if (plan_resident_shape(c, n, &t[i], q) == FPSX_OK) out.push_back(q);
+  }
+  for (int B : {256, 1024}) {
+    Plan q;
+    if (plan_stream(c, n, dim, tie, B, 0, false, q) == FPSX_OK) out.push_back(q);
+  }
+  return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -448,6 +504,60 @@ FPSX_EXPORT int fpsx_plan_info(size_t n_points, size_t dim, int tie, int *kind, 
   if (ppt) *ppt = p.k->ppt;
   if (grid) *grid = (int)p.grid;
   if (chunk) *chunk = (int)p.chunk;
+  return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_autotune(const float *d_points, size_t stride_point, size_t stride_dim,
+                              size_t n_points, size_t dim, int tie, size_t passes, void *stream,
+                              int *chosen_block, int *chosen_ppt, int *chosen_grid) {
+  t_err.clear();
+  if (!d_points || n_points == 0 || dim == 0) return fail(FPSX_EINVAL, "bad arguments");
+  if (n_points >= (size_t(1) << 31)) return fail(FPSX_ETOOBIG, "n_points must be < 2^31");
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+  const size_t m = std::max<size_t>(2, std::min(passes ? passes : 64, n_points));
+  HIP_TRY(c.starts.reserve(sizeof(uint64_t)));
+  HIP_TRY(c.out.reserve(m * sizeof(uint64_t)));
+  launch_fill_u64((uint64_t *)c.starts.p, 1, 0, st);
+  std::vector<Plan> cands = candidates(c, n_points, dim, tie);
+  if (cands.empty()) return fail(FPSX_EINVAL, "no feasible plan");
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double best = 1e30;
+  const Plan *bp = nullptr;
+  for (const Plan &q : cands) {
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      HIP_TRY(hipEventRecord(e0, st));
+      rc = launch_fps(c, q, d_points, stride_point, stride_dim, n_points, dim, m,
+                      (const uint64_t *)c.starts.p, 1, (uint64_t *)c.out.p, nullptr, INFINITY, st);
+      if (rc) return rc;
+      HIP_TRY(hipEventRecord(e1, st));
+      HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      tmin = std::min(tmin, ms);
+    }
+    HIP_TRY(hipMemcpyAsync(c.host_status, c.status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (*c.host_status != 0) continue;  // a candidate that failed is never chosen
+    if (tmin < best) {
+      best = tmin;
+      bp = &q;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (!bp) return fail(FPSX_EHIP, "every candidate plan failed");
+  c.tuned[std::make_tuple(n_points, dim, tie)] = std::make_pair(bp->k, bp->grid);
+  if (chosen_block) *chosen_block = (int)bp->block;
+  if (chosen_ppt) *chosen_ppt = bp->k->ppt;
+  if (chosen_grid) *chosen_grid = (int)bp->grid;
   return FPSX_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -16,11 +17,13 @@
 
 namespace fpsx {
 
-// Grow-only device buffer.
+// Grow-only device buffer.  Buffers of the per-device contexts live until the
+// process ends; owners with a shorter life call free().
 struct DevBuf {
   void *p = nullptr;
   size_t cap = 0;
   hipError_t reserve(size_t bytes);
+  void free();
   ~DevBuf();
 };
 
@@ -37,6 +40,8 @@ struct Ctx {
   DevBuf pts, rows, starts, out, xchg, status, dmin;
   std::map<std::pair<const void *, size_t>, int> occ;  // co-resident workgroups per kernel
   int max_blocks(const KernelEntry *k, size_t shmem);
+  // Autotuned plans: (n_points, dim, tie) -> (kernel entry, grid).
+  std::map<std::tuple<size_t, size_t, int>, std::pair<const KernelEntry *, uint32_t>> tuned;
   // Optional per-launch timing of the FPS kernel (HIP events on its stream).
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

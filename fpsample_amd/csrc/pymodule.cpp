@@ -15,7 +15,9 @@
 //     the end of the array (src/lib.cpp:172).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
 
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -152,6 +154,57 @@ py::array_t<size_t> bucket_fps_kdline_sampling_py(CArray points, size_t n_sample
   return out;
 }
 
+// Batched form (no counterpart in the reference): every cloud sampled as
+// fps_sampling would sample it alone, all in one persistent GPU launch.
+py::list fps_sampling_batch_py(py::list clouds, std::vector<size_t> n_samples,
+                               std::vector<size_t> starts) {
+  const size_t nc = clouds.size();
+  if (nc == 0) throw py::value_error("clouds must not be empty");
+  if (n_samples.size() != nc || starts.size() != nc)
+    throw py::value_error("n_samples and start_idx must have one entry per cloud");
+  std::vector<CArray> arrs;
+  arrs.reserve(nc);
+  std::vector<uint64_t> offsets(nc + 1, 0);
+  py::ssize_t dim = -1;
+  for (size_t c = 0; c < nc; ++c) {
+    CArray a = CArray::ensure(clouds[c]);
+    if (!a) throw py::value_error("cloud " + std::to_string(c) + " is not convertible to float32");
+    if (a.ndim() != 2) throw py::value_error("every cloud must be a 2D array");
+    if (dim < 0) dim = a.shape(1);
+    if (a.shape(1) != dim || dim == 0)
+      throw py::value_error("every cloud must have the same, non-zero number of columns");
+    offsets[c + 1] = offsets[c] + (uint64_t)a.shape(0);
+    arrs.push_back(a);
+  }
+  std::vector<float> rows(offsets[nc] * (size_t)dim);
+  for (size_t c = 0; c < nc; ++c)
+    std::memcpy(rows.data() + offsets[c] * dim, arrs[c].data(),
+                (size_t)arrs[c].shape(0) * dim * sizeof(float));
+  size_t total_m = 0;
+  for (size_t v : n_samples) total_m += v;
+  std::vector<uint64_t> ns(n_samples.begin(), n_samples.end());
+  std::vector<uint64_t> st(starts.begin(), starts.end());
+  std::vector<uint64_t> out(total_m);
+  int rc;
+  {
+    py::gil_scoped_release nogil;
+    rc = fpsx_fps_batch(rows.data(), (size_t)dim, offsets.data(), nc, ns.data(), st.data(),
+                        out.data());
+  }
+  if (rc == FPSX_EINVAL || rc == FPSX_EBADSTART || rc == FPSX_EBADDIM)
+    throw py::value_error(fpsx_last_error());
+  if (rc != FPSX_OK) raise_rc(rc, "fps_sampling_batch");
+  py::list res;
+  size_t off = 0;
+  for (size_t c = 0; c < nc; ++c) {
+    py::array_t<size_t> a(n_samples[c]);
+    std::memcpy(a.mutable_data(), out.data() + off, n_samples[c] * sizeof(uint64_t));
+    off += n_samples[c];
+    res.append(a);
+  }
+  return res;
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_fpsample, m) {
@@ -189,6 +242,17 @@ PYBIND11_MODULE(_fpsample, m) {
                 start_idx (int): position of the first pick in the tree order.
             Returns:
                 np.ndarray[uint64]: sampled point indices.
+    )pbdoc");
+
+  m.def("_fps_sampling_batch", &fps_sampling_batch_py, py::arg("clouds"), py::arg("n_samples"),
+        py::arg("start_idx"), R"pbdoc(
+            Batched FPS: one persistent GPU launch over many clouds.
+            Args:
+                clouds (list[np.ndarray[float32, 2D]]): clouds with the same number of columns.
+                n_samples (list[int]): picks per cloud.
+                start_idx (list[int]): first pick per cloud.
+            Returns:
+                list[np.ndarray[uint64]]: per-cloud picks, each equal to _fps_sampling on that cloud.
     )pbdoc");
 
   m.attr("__version__") = fpsx_version();

@@ -63,6 +63,9 @@ def lib() -> ctypes.CDLL:
                                                        ctypes.c_int, ctypes.c_float, vp]
         L.fpsx_fps_sampling_device_strided.restype = ctypes.c_int
         L.fpsx_device_status.argtypes = [vp]
+        ip_ = ctypes.POINTER(ctypes.c_int)
+        L.fpsx_autotune.argtypes = [vp, sz, sz, sz, sz, ctypes.c_int, sz, vp, ip_, ip_, ip_]
+        L.fpsx_autotune.restype = ctypes.c_int
         L.fpsx_timing.argtypes = [ctypes.c_int]
         L.fpsx_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
         ip = ctypes.POINTER(ctypes.c_int)
@@ -192,6 +195,17 @@ def fps_sampling_tensor(points, n_samples: int, start_idx: Union[int, Sequence[i
     synchronize()
     d_start.free()
     return out, out_pts
+
+
+def autotune(points, stride_point: int, stride_dim: int, n_points: int, dim: int,
+             tie: int = TIE_HIGHEST, passes: int = 64, stream: Optional[int] = None) -> dict:
+    """Time every feasible kernel shape on this cloud and keep the fastest for
+    clouds of the same (n_points, dim, tie); returns the chosen shape."""
+    b, p, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = lib().fpsx_autotune(_addr(points), stride_point, stride_dim, n_points, dim, tie, passes,
+                             stream, ctypes.byref(b), ctypes.byref(p), ctypes.byref(g))
+    _raise(rc, "fpsx_autotune")
+    return dict(block=b.value, ppt=p.value, grid=g.value)
 
 
 def device_status(stream: Optional[int] = None) -> None:

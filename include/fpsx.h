@@ -66,6 +66,18 @@ int fpsx_plan_info(size_t n_points, size_t dim, int tie, int *kind, int *block, 
                    int *grid, int *chunk);
 
 /*
+ * Autotuner: times every feasible kernel shape (threads per workgroup, points
+ * per thread, workgroups) for `passes` passes (0: 64) on the given device
+ * cloud and keeps the fastest for clouds of exactly this (n_points, dim,
+ * tie) on the current device; later calls plan with it.  The chosen shape is
+ * returned through the optional pointers.  Without tuning the planner uses a
+ * per-pass time model.
+ */
+int fpsx_autotune(const float *d_points, size_t stride_point, size_t stride_dim, size_t n_points,
+                  size_t dim, int tie, size_t passes, void *stream, int *chosen_block,
+                  int *chosen_ppt, int *chosen_grid);
+
+/*
  * Kernel timing on the current device: fpsx_timing(1) makes every later FPS
  * launch record a HIP event pair around the FPS kernel on the stream it is
  * launched on (and forgets earlier records); fpsx_timing_read waits for the
@@ -121,6 +133,33 @@ int fpsx_fps_sampling_device_strided(const float *d_points, size_t stride_point,
                                      size_t n_points, size_t dim, size_t n_samples,
                                      const uint64_t *d_start, size_t n_start, uint64_t *d_out,
                                      float *d_out_points, int tie, float init, void *stream);
+
+/*
+ * Batched FPS: many independent clouds sampled in one persistent launch.
+ * The clouds are concatenated in one SoA device array (coordinate j of point p
+ * at d_points[j*ld + p]); cloud c holds points offsets[c] .. offsets[c+1]-1
+ * (n_clouds + 1 host offsets) and gets n_samples[c] picks starting at its own
+ * index start_idx[c] (host arrays).  Picks land in d_out, cloud after cloud
+ * (cloud c at sum of earlier n_samples), each equal to fpsx_fps_sampling on
+ * that cloud alone.  A batch object owns its device tables; flags:
+ *   FPSX_BATCH_GRAPH     capture the launch sequence in a HIP graph, replayed by run
+ *   FPSX_BATCH_AUTOTUNE  time every batch-kernel shape once at creation, keep the fastest
+ * Clouds of a dimension with no batch kernel run one launch per cloud.
+ */
+#define FPSX_BATCH_GRAPH 1
+#define FPSX_BATCH_AUTOTUNE 2
+typedef struct fpsx_batch fpsx_batch;
+int fpsx_batch_create(const float *d_points, size_t ld, size_t dim, const uint64_t *offsets,
+                      size_t n_clouds, const uint64_t *n_samples, const uint64_t *start_idx,
+                      uint64_t *d_out, int flags, fpsx_batch **out);
+int fpsx_batch_run(fpsx_batch *b, void *stream);    /* enqueue one run (graph replay if captured) */
+int fpsx_batch_status(fpsx_batch *b, void *stream); /* wait, report the kernel status */
+int fpsx_batch_info(fpsx_batch *b, int *block, int *ppt, int *grid, int *members, int *graph);
+void fpsx_batch_destroy(fpsx_batch *b);
+
+/* Host convenience: row-major concatenated clouds in, picks out (synchronous). */
+int fpsx_fps_batch(const float *points, size_t dim, const uint64_t *offsets, size_t n_clouds,
+                   const uint64_t *n_samples, const uint64_t *start_idx, uint64_t *out);
 
 /*
  * Bucket FPS entry with the exact signature and return codes of the

@@ -23,7 +23,7 @@ HIGHEST, LOWEST = 0, 1
 class CApi:
     def __init__(self, path: str = LIB):
         if not os.path.exists(path):
-            raise FileNotFoundError(f"{path} is not built (python -m fpsample_amd.build)")
+            raise FileNotFoundError(f"{path} is not built (python fpsample_amd/build.py)")
         L = ctypes.CDLL(path)
         L.fpsx_last_error.restype = ctypes.c_char_p
         L.fpsx_version.restype = ctypes.c_char_p

@@ -1,0 +1,93 @@
+"""GPU parity of the batched path: every cloud of a batch equals the oracle's
+fps_sampling of that cloud alone, for every batch-kernel shape, with teams of
+several workgroups, through HIP-graph replays and through the Python API."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def ragged(seed, k, lo, hi, dim=3):
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(lo), np.log(hi), k)).astype(np.int64)
+    clouds = []
+    for i, n in enumerate(ns):
+        if i % 3 == 0:
+            pc = rng.random((n, dim))
+        elif i % 3 == 1:
+            v = rng.standard_normal((n, dim))
+            pc = v / np.linalg.norm(v, axis=1, keepdims=True) + 0.01 * rng.standard_normal((n, dim))
+        else:
+            pc = rng.integers(0, 4, (n, dim))  # ties
+        clouds.append(pc.astype(np.float32))
+    return clouds
+
+
+def oracle_all(clouds, ms, starts):
+    return [O.fps(pc, m, s) for pc, m, s in zip(clouds, ms, starts)]
+
+
+@pytest.mark.parametrize("shape", ["1024:16", "1024:8", "256:32", "256:16", "256:8", "256:4"])
+def test_batch_shapes(gpu, monkeypatch, shape):
+    import fpsample_amd as F
+
+    monkeypatch.setenv("FPSX_BATCH_SHAPE", shape)
+    clouds = ragged(1, 24, 50, 20000)
+    ms = [max(1, len(pc) // 8) for pc in clouds]
+    starts = [len(pc) // 3 for pc in clouds]
+    log(f"shape {shape}: sizes {[len(c) for c in clouds]}")
+    got = F.fps_sampling_batch(clouds, ms, starts)
+    for i, (g, w) in enumerate(zip(got, oracle_all(clouds, ms, starts))):
+        np.testing.assert_array_equal(g, w, err_msg=f"shape {shape} cloud {i} n={len(clouds[i])}")
+
+
+def test_batch_teams_large_clouds(gpu, monkeypatch):
+    import fpsample_amd as F
+
+    # 256:16 holds 4096 points per workgroup: clouds of 3k..60k need teams of 1..15
+    monkeypatch.setenv("FPSX_BATCH_SHAPE", "256:16")
+    clouds = ragged(2, 10, 3000, 60000)
+    ms = [min(len(pc), 600) for pc in clouds]
+    got = F.fps_sampling_batch(clouds, ms, 0)
+    for i, (g, w) in enumerate(zip(got, oracle_all(clouds, ms, [0] * len(clouds)))):
+        np.testing.assert_array_equal(g, w, err_msg=f"cloud {i} n={len(clouds[i])}")
+
+
+def test_batch_other_dims_fall_back(gpu):
+    import fpsample_amd as F
+
+    for d in (1, 2, 5):
+        clouds = ragged(3 + d, 6, 20, 3000, dim=d)
+        ms = [max(1, len(pc) // 4) for pc in clouds]
+        got = F.fps_sampling_batch(clouds, ms, 1)
+        for g, w in zip(got, oracle_all(clouds, ms, [1] * len(clouds))):
+            np.testing.assert_array_equal(g, w)
+
+
+def test_batch_graph_replays(gpu):
+    from fpsample_amd import device as D
+    from fpsample_amd import batch as Bt
+
+    clouds = ragged(4, 40, 100, 30000)
+    ms = [max(1, len(pc) // 4) for pc in clouds]
+    starts = [0] * len(clouds)
+    want = np.concatenate(oracle_all(clouds, ms, starts))
+    b = Bt.DeviceBatch(clouds, ms, starts, graph=True)
+    for rep in range(4):
+        b.run()
+        b.status()
+        np.testing.assert_array_equal(b.picks_flat(), want, err_msg=f"replay {rep}")
+        # eager work between replays: a single-cloud run on the same device
+        rc_pc = clouds[rep]
+        got1, _ = D.fps_sampling_tensor(D.DeviceArray.from_numpy(rc_pc), ms[rep], 0)
+        np.testing.assert_array_equal(got1.numpy(), O.fps(rc_pc, ms[rep], 0))
+    b.close()
