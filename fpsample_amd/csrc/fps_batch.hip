@@ -42,7 +42,7 @@ __device__ __forceinline__ bool run_job(const BatchParams &a, const BatchJob &J,
     const uint32_t i = lo + (uint32_t)(k * B + tid);
     const bool live = k < kmax && i < hi;
 #pragma unroll
-    for (int d = 0; d < D; ++d) c[k][d] = live ? pts[(size_t)d * a.ld + i] : 0.0f;
+    for (int d = 0; d < D; ++d) c[k][d] = live ? pts[(size_t)d * J.ld + i] : 0.0f;
     dm[k] = live ? (J.s_begin == 0 ? a.init : a.state[J.state_off + i]) : -1.0f;
   }
   uint32_t pidx = (J.s_begin == 0) ? (uint32_t)J.start : (uint32_t)a.out[J.out_off + J.s_begin - 1];
@@ -52,7 +52,7 @@ __device__ __forceinline__ bool run_job(const BatchParams &a, const BatchJob &J,
   }
   float pv[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) pv[d] = pts[(size_t)d * a.ld + pidx];
+  for (int d = 0; d < D; ++d) pv[d] = pts[(size_t)d * J.ld + pidx];
   if (J.s_begin == 0 && r == 0 && tid == B - 64) a.out[J.out_off] = pidx;
 
   uint64_t *area = a.xchg + (size_t)J.area * kTeamXchgWords;

@@ -13,7 +13,8 @@ constexpr size_t kTeamXchgWords = 2 * 16 * kMaxTeam;  // 2 buffers x kSlotWords 
 
 // One cloud of a batch (and the passes this launch makes on it).
 struct BatchJob {
-  uint64_t pt_off;     // first point of the cloud in the concatenated SoA arrays
+  uint64_t pt_off;     // the cloud's SoA block starts at pts + pt_off
+  uint64_t ld;         // coordinate d of point i at pts[pt_off + d * ld + i]
   uint64_t out_off;    // first pick of the cloud in out[]
   uint64_t state_off;  // first saved min-distance of the cloud in state[] (chunked runs)
   uint64_t start;      // pick 0
@@ -28,8 +29,7 @@ struct BatchJob {
 };
 
 struct BatchParams {
-  const float *pts;          // concatenated SoA: coordinate d of point p at pts[d * ld + p]
-  uint64_t ld;
+  const float *pts;          // base of every job's SoA block (BatchJob::pt_off, ::ld)
   const BatchJob *jobs;
   const uint32_t *member_job;  // claim slot -> job
   uint32_t total_members;

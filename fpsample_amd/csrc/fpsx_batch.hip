@@ -49,6 +49,7 @@ int build_jobs(const JobSpec *specs, size_t n_specs, const KernelEntry *k, uint3
     BatchJob &J = js[c];
     std::memset(&J, 0, sizeof(J));
     J.pt_off = sp.pt_off;
+    J.ld = sp.ld;
     J.out_off = sp.out_off;
     J.state_off = sp.state_off;
     J.start = sp.start;
@@ -91,7 +92,7 @@ int build_jobs(const JobSpec *specs, size_t n_specs, const KernelEntry *k, uint3
 
 int BatchRun::prepare(Ctx &c, const KernelEntry *kernel, const std::vector<BatchJob> &js,
                       const std::vector<uint32_t> &mj, uint32_t n_areas, const float *d_pts,
-                      uint64_t ld, uint64_t *d_out, float *d_state) {
+                      uint64_t *d_out, float *d_state) {
   k = kernel;
   jobs = js;
   member_job = mj;
@@ -115,7 +116,6 @@ int BatchRun::prepare(Ctx &c, const KernelEntry *kernel, const std::vector<Batch
                     hipMemcpyHostToDevice));
   std::memset(&params, 0, sizeof(params));
   params.pts = d_pts;
-  params.ld = ld;
   params.jobs = (const BatchJob *)d_jobs.p;
   params.member_job = (const uint32_t *)d_member_job.p;
   params.total_members = members;
@@ -163,7 +163,7 @@ struct fpsx_batch {
 };
 
 static int validate_batch(const uint64_t *offsets, size_t n_clouds, const uint64_t *n_samples,
-                          const uint64_t *start_idx, std::vector<JobSpec> &specs) {
+                          const uint64_t *start_idx, uint64_t ld, std::vector<JobSpec> &specs) {
   if (!offsets || !n_samples || !start_idx || n_clouds == 0)
     return fail(FPSX_EINVAL, "empty batch or null pointer");
   specs.resize(n_clouds);
@@ -179,6 +179,7 @@ static int validate_batch(const uint64_t *offsets, size_t n_clouds, const uint64
       return fail(FPSX_EBADSTART, "cloud " + std::to_string(c) + ": start index out of range");
     JobSpec &s = specs[c];
     s.pt_off = offsets[c];
+    s.ld = ld;
     s.out_off = out_off;
     s.state_off = 0;
     s.start = start_idx[c];
@@ -197,7 +198,7 @@ static int batch_prepare_kernel(fpsx_batch *b, const KernelEntry *k) {
   uint32_t areas = 0;
   int rc = build_jobs(b->specs.data(), b->specs.size(), k, 0, jobs, mj, &areas);
   if (rc) return rc;
-  return b->run.prepare(*b->ctx, k, jobs, mj, areas, b->d_pts, b->ld, b->d_out, nullptr);
+  return b->run.prepare(*b->ctx, k, jobs, mj, areas, b->d_pts, b->d_out, nullptr);
 }
 
 FPSX_EXPORT int fpsx_batch_create(const float *d_points, size_t ld, size_t dim,
@@ -209,7 +210,7 @@ FPSX_EXPORT int fpsx_batch_create(const float *d_points, size_t ld, size_t dim,
   if (dim == 0) return fail(FPSX_EBADDIM, "dim must be >= 1");
   if (!d_points || !d_out) return fail(FPSX_EINVAL, "null pointer argument");
   std::unique_ptr<fpsx_batch> b(new fpsx_batch());
-  int rc = validate_batch(offsets, n_clouds, n_samples, start_idx, b->specs);
+  int rc = validate_batch(offsets, n_clouds, n_samples, start_idx, ld, b->specs);
   if (rc) return rc;
   if (ld < offsets[n_clouds]) return fail(FPSX_EINVAL, "ld must be >= the total point count");
   rc = get_ctx(&b->ctx);
@@ -363,7 +364,7 @@ FPSX_EXPORT int fpsx_fps_batch(const float *points, size_t dim, const uint64_t *
   if (dim == 0) return fail(FPSX_EBADDIM, "dim must be >= 1");
   if (!points || !out) return fail(FPSX_EINVAL, "null pointer argument");
   std::vector<JobSpec> specs;
-  int rc = validate_batch(offsets, n_clouds, n_samples, start_idx, specs);
+  int rc = validate_batch(offsets, n_clouds, n_samples, start_idx, offsets[n_clouds], specs);
   if (rc) return rc;
   const size_t total = offsets[n_clouds];
   size_t total_m = 0;

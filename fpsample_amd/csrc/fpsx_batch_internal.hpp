@@ -12,7 +12,7 @@ namespace fpsx {
 
 // One cloud's work for a launch, before team sizes are chosen.
 struct JobSpec {
-  uint64_t pt_off, out_off, state_off, start;
+  uint64_t pt_off, ld, out_off, state_off, start;
   uint32_t n, m, s_begin, s_end;
 };
 
@@ -33,7 +33,7 @@ struct BatchRun {
   BatchParams params;
   // Upload tables for this job list (host-synchronous) and fill params.
   int prepare(Ctx &c, const KernelEntry *kernel, const std::vector<BatchJob> &js,
-              const std::vector<uint32_t> &mj, uint32_t n_areas, const float *d_pts, uint64_t ld,
+              const std::vector<uint32_t> &mj, uint32_t n_areas, const float *d_pts,
               uint64_t *d_out, float *d_state);
   // Clear counters / exchange areas / status and launch (graph-capturable).
   int enqueue(hipStream_t st);

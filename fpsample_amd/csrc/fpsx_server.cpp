@@ -1,0 +1,422 @@
+// fpsx_server.cpp -- FPS serving loop with HBM-budget admission and
+// continuous batching (host runtime, HIP API only; the kernel is fps_batch).
+//
+// Requests (one cloud each, any size) are submitted from any thread.  A
+// scheduler thread owns the device:
+//   admit    -- pending requests enter, oldest first, while their device
+//               footprint fits the HBM budget (a device arena allocated once;
+//               first-fit ranges, no per-request hipMalloc) and their team of
+//               workgroups fits the co-resident grid; their points are copied
+//               on a separate stream, overlapping the running kernel;
+//   launch   -- one fps_batch launch advances every admitted, uploaded request
+//               by up to `chunk` passes (min-distances saved in the arena
+//               between launches), so requests that arrive while others run
+//               join at the next launch;
+//   retire   -- finished requests copy their picks back, free their range and
+//               wake their waiters.
+// Each request's picks equal fps_sampling on its cloud alone: a job's result
+// never depends on what else shares the launch (exact min and argmax, ties to
+// the highest index), and a chunk boundary only saves and restores state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "fps_batch.hpp"
+#include "fpsx.h"
+#include "fpsx_batch_internal.hpp"
+#include "fpsx_internal.hpp"
+
+#define FPSX_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace fpsx {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double secs_since(Clock::time_point t0, Clock::time_point t) {
+  return std::chrono::duration<double>(t - t0).count();
+}
+
+// First-fit allocator over [0, size) in 256-byte units.
+class Arena {
+ public:
+  explicit Arena(size_t bytes) { free_[0] = bytes / kAlign; units_ = bytes / kAlign; }
+  bool alloc(size_t bytes, size_t *off) {
+    const size_t need = (bytes + kAlign - 1) / kAlign;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < need) continue;
+      *off = it->first * kAlign;
+      const size_t start = it->first + need, left = it->second - need;
+      free_.erase(it);
+      if (left) free_[start] = left;
+      used_ += need;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    const size_t u = off / kAlign, n = (bytes + kAlign - 1) / kAlign;
+    used_ -= n;
+    auto it = free_.emplace(u, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t used_bytes() const { return used_ * kAlign; }
+  static constexpr size_t kAlign = 256;
+
+ private:
+  std::map<size_t, size_t> free_;  // start unit -> length in units
+  size_t units_ = 0, used_ = 0;
+};
+
+size_t align_up(size_t v) { return (v + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign; }
+
+struct Request {
+  uint64_t id = 0;
+  std::vector<float> soa;  // host copy of the cloud, SoA (dim x n)
+  uint32_t n = 0, m = 0, dim = 0;
+  uint64_t start = 0;
+  // device placement (byte offsets into the arena)
+  size_t off = 0, bytes = 0, pts_off = 0, state_off = 0, out_off = 0;
+  uint32_t team = 1;
+  uint32_t next = 0;  // next pick to compute (0 = not started)
+  hipEvent_t uploaded = nullptr;
+  bool ready = false, done = false;
+  int error = FPSX_OK;
+  std::vector<uint64_t> picks;
+  Clock::time_point t_submit, t_admit, t_done;
+};
+
+}  // namespace
+
+}  // namespace fpsx
+
+using namespace fpsx;
+
+struct fpsx_server {
+  Ctx *ctx = nullptr;
+  const KernelEntry *k = nullptr;
+  uint32_t chunk = 256;
+  size_t budget = 0;
+  void *arena_base = nullptr;
+  std::unique_ptr<Arena> arena;
+  uint32_t grid_cap = 0;       // co-resident workgroups of the batch kernel
+  hipStream_t compute = nullptr, copy = nullptr;
+  BatchRun run;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> pending;
+  std::vector<std::shared_ptr<Request>> active;
+  std::map<uint64_t, std::shared_ptr<Request>> all;
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::thread thread;
+  Clock::time_point t0 = Clock::now();
+  // counters
+  uint64_t launches = 0, completed = 0, samples_done = 0;
+  double busy_s = 0;
+  int fatal = FPSX_OK;
+  std::string fatal_msg;
+
+  uint32_t team_of(uint32_t n) const {
+    const uint32_t cap = (uint32_t)(k->block * k->ppt);
+    const uint32_t t = (n + cap - 1) / cap;
+    return std::max(1u, t);
+  }
+  size_t footprint(const Request &r) const {
+    return align_up((size_t)r.n * r.dim * sizeof(float)) + align_up((size_t)r.n * sizeof(float)) +
+           align_up((size_t)r.m * sizeof(uint64_t));
+  }
+  void loop();
+  int step();  // one admit / launch / retire round; caller does not hold mu
+};
+
+int fpsx_server::step() {
+  std::vector<std::shared_ptr<Request>> admitted;
+  {
+    std::unique_lock<std::mutex> lk(mu);
+    uint32_t members = 0;
+    for (auto &r : active) members += r->team;
+    while (!pending.empty()) {
+      auto &r = pending.front();
+      if (members + r->team > grid_cap) break;
+      size_t off;
+      if (!arena->alloc(r->bytes, &off)) break;  // wait for memory to free up
+      r->off = off;
+      r->pts_off = off;
+      r->state_off = off + align_up((size_t)r->n * r->dim * sizeof(float));
+      r->out_off = r->state_off + align_up((size_t)r->n * sizeof(float));
+      r->t_admit = Clock::now();
+      members += r->team;
+      admitted.push_back(r);
+      active.push_back(r);
+      pending.pop_front();
+    }
+  }
+  // Upload newly admitted clouds on the copy stream (overlaps the kernel).
+  for (auto &r : admitted) {
+    char *base = (char *)arena_base;
+    hipError_t e = hipMemcpyAsync(base + r->pts_off, r->soa.data(), r->soa.size() * sizeof(float),
+                                  hipMemcpyHostToDevice, copy);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(r->uploaded, copy);
+    if (e != hipSuccess) return fail(FPSX_EHIP, std::string("upload: ") + hipGetErrorString(e));
+  }
+  // Jobs of this launch: every uploaded request, next chunk of passes.
+  std::vector<std::shared_ptr<Request>> in_launch;
+  std::vector<JobSpec> specs;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &r : active) {
+      if (!r->ready) {
+        if (hipEventQuery(r->uploaded) != hipSuccess) continue;
+        r->ready = true;
+        r->soa.clear();
+        r->soa.shrink_to_fit();
+      }
+      JobSpec s;
+      s.pt_off = r->pts_off / sizeof(float);
+      s.ld = r->n;
+      s.out_off = r->out_off / sizeof(uint64_t);
+      s.state_off = r->state_off / sizeof(float);
+      s.start = r->start;
+      s.n = r->n;
+      s.m = r->m;
+      s.s_begin = r->next;
+      s.s_end = std::min<uint32_t>(r->m, (r->next == 0 ? 1u : r->next) + chunk);
+      specs.push_back(s);
+      in_launch.push_back(r);
+    }
+  }
+  if (specs.empty()) {
+    // Everything admitted is still uploading: wait for the copies instead of
+    // spinning.
+    hipError_t e = hipStreamSynchronize(copy);
+    if (e != hipSuccess) return fail(FPSX_EHIP, std::string("upload: ") + hipGetErrorString(e));
+    return FPSX_OK;
+  }
+  std::vector<BatchJob> jobs;
+  std::vector<uint32_t> mj;
+  uint32_t areas = 0;
+  int rc = build_jobs(specs.data(), specs.size(), k, 0, jobs, mj, &areas);
+  if (rc) return rc;
+  // pt/out/state offsets are element offsets from the arena base for all three views
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rc = run.prepare(*ctx, k, jobs, mj, areas, (const float *)arena_base, (uint64_t *)arena_base,
+                     (float *)arena_base);
+  }
+  if (rc) return rc;
+  const auto t_l0 = Clock::now();
+  rc = run.enqueue(compute);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(compute);
+  if (e != hipSuccess) return fail(FPSX_EHIP, std::string("serve launch: ") + hipGetErrorString(e));
+  busy_s += secs_since(t_l0, Clock::now());
+  uint32_t st = 0;
+  e = hipMemcpy(&st, run.d_status.p, sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(FPSX_EHIP, std::string("status: ") + hipGetErrorString(e));
+  if (st) return status_code(st);
+  ++launches;
+  // Retire finished requests.
+  std::vector<std::shared_ptr<Request>> finished;
+  for (size_t i = 0; i < in_launch.size(); ++i) {
+    auto &r = in_launch[i];
+    r->next = specs[i].s_end;
+    if (r->next >= r->m) finished.push_back(r);
+  }
+  for (auto &r : finished) {
+    r->picks.resize(r->m);
+    e = hipMemcpy(r->picks.data(), (char *)arena_base + r->out_off, r->m * sizeof(uint64_t),
+                  hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r->error = FPSX_EHIP;
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    const auto now = Clock::now();
+    for (auto &r : finished) {
+      r->done = true;
+      r->t_done = now;
+      arena->release(r->off, r->bytes);
+      if (r->uploaded) hipEventDestroy(r->uploaded);
+      r->uploaded = nullptr;
+      active.erase(std::find(active.begin(), active.end(), r));
+      ++completed;
+      samples_done += r->m;
+    }
+  }
+  if (!finished.empty()) cv_done.notify_all();
+  return FPSX_OK;
+}
+
+void fpsx_server::loop() {
+  hipSetDevice(ctx->device);
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
+      if (stop && pending.empty() && active.empty()) return;
+    }
+    const int rc = step();
+    if (rc != FPSX_OK) {
+      std::lock_guard<std::mutex> lk(mu);
+      fatal = rc;
+      fatal_msg = fpsx_last_error();
+      for (auto &r : active) {
+        r->done = true;
+        r->error = rc;
+      }
+      for (auto &r : pending) {
+        r->done = true;
+        r->error = rc;
+      }
+      active.clear();
+      pending.clear();
+      cv_done.notify_all();
+      return;
+    }
+  }
+}
+
+FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passes, int block,
+                                   int ppt, fpsx_server **out) {
+  if (!out) return fail(FPSX_EINVAL, "null output handle");
+  *out = nullptr;
+  if (hbm_budget_bytes < (1u << 20)) return fail(FPSX_EINVAL, "budget must be >= 1 MiB");
+  std::unique_ptr<fpsx_server> s(new fpsx_server());
+  int rc = get_ctx(&s->ctx);
+  if (rc) return rc;
+  size_t cnt;
+  const KernelEntry *t = batch_table(&cnt);
+  const int B = block ? block : 1024, P = ppt ? ppt : 16;
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].dim == 3 && t[i].block == B && t[i].ppt == P) s->k = &t[i];
+  if (!s->k) return fail(FPSX_EINVAL, "no batch kernel with that shape");
+  {
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    s->grid_cap = (uint32_t)s->ctx->max_blocks(s->k, 0);
+  }
+  if (s->grid_cap == 0) return fail(FPSX_EHIP, "batch kernel cannot be resident");
+  s->chunk = chunk_passes ? chunk_passes : 256;
+  s->budget = hbm_budget_bytes / Arena::kAlign * Arena::kAlign;
+  hipError_t e = hipMalloc(&s->arena_base, s->budget);
+  if (e != hipSuccess) return fail(FPSX_ENOMEM, "arena: " + std::string(hipGetErrorString(e)));
+  s->arena.reset(new Arena(s->budget));
+  e = hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(FPSX_EHIP, "streams: " + std::string(hipGetErrorString(e)));
+  fpsx_server *raw = s.get();
+  raw->thread = std::thread([raw] { raw->loop(); });
+  *out = s.release();
+  return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_server_submit(fpsx_server *s, const float *points, size_t n_points, size_t dim,
+                                   size_t n_samples, uint64_t start_idx, uint64_t *request_id) {
+  if (!s || !points || !request_id) return fail(FPSX_EINVAL, "null argument");
+  if (dim != 3) return fail(FPSX_EBADDIM, "the serving loop takes 3-D clouds");
+  if (n_points == 0 || n_points >= (size_t(1) << 31)) return fail(FPSX_ETOOBIG, "bad n_points");
+  if (n_samples == 0 || n_samples > n_points) return fail(FPSX_EINVAL, "n_samples must be in [1, n]");
+  if (start_idx >= n_points) return fail(FPSX_EBADSTART, "start index out of range");
+  auto r = std::make_shared<Request>();
+  r->n = (uint32_t)n_points;
+  r->m = (uint32_t)n_samples;
+  r->dim = (uint32_t)dim;
+  r->start = start_idx;
+  r->team = s->team_of(r->n);
+  r->bytes = s->footprint(*r);
+  if (r->bytes > s->budget)
+    return fail(FPSX_EBUDGET, "request needs " + std::to_string(r->bytes) +
+                                  " bytes of HBM, more than the whole budget");
+  if (r->team > s->grid_cap || r->team > kMaxTeam || r->team > (uint32_t)s->k->block)
+    return fail(FPSX_ETOOBIG, "cloud too large for the serving kernel's team limit");
+  r->soa.resize(n_points * dim);
+  for (size_t i = 0; i < n_points; ++i)
+    for (size_t d = 0; d < dim; ++d) r->soa[d * n_points + i] = points[i * dim + d];
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->fatal) return fail(s->fatal, "server stopped: " + s->fatal_msg);
+    r->id = s->next_id++;
+    s->all[r->id] = r;
+    s->pending.push_back(r);
+  }
+  s->cv_work.notify_one();
+  *request_id = r->id;
+  return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_server_wait(fpsx_server *s, uint64_t request_id, uint64_t *out,
+                                 double *latency_s) {
+  if (!s) return fail(FPSX_EINVAL, "null server");
+  std::shared_ptr<Request> r;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(request_id);
+  if (it == s->all.end()) return fail(FPSX_EINVAL, "unknown request id");
+  r = it->second;
+  s->cv_done.wait(lk, [&] { return r->done; });
+  s->all.erase(request_id);
+  if (r->error) return fail(r->error, "request failed: " + s->fatal_msg);
+  if (out) std::memcpy(out, r->picks.data(), r->m * sizeof(uint64_t));
+  if (latency_s) *latency_s = secs_since(r->t_submit, r->t_done);
+  return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_server_stats(fpsx_server *s, uint64_t *launches, uint64_t *completed,
+                                  uint64_t *samples, double *busy_s, size_t *hbm_in_use,
+                                  size_t *pending, size_t *active) {
+  if (!s) return fail(FPSX_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (launches) *launches = s->launches;
+  if (completed) *completed = s->completed;
+  if (samples) *samples = s->samples_done;
+  if (busy_s) *busy_s = s->busy_s;
+  if (hbm_in_use) *hbm_in_use = s->arena->used_bytes();
+  if (pending) *pending = s->pending.size();
+  if (active) *active = s->active.size();
+  return s->fatal;
+}
+
+FPSX_EXPORT void fpsx_server_destroy(fpsx_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->thread.joinable()) s->thread.join();
+  hipStreamSynchronize(s->compute);
+  hipStreamSynchronize(s->copy);
+  s->run.d_jobs.free();
+  s->run.d_member_job.free();
+  s->run.d_counters.free();
+  s->run.d_xchg.free();
+  s->run.d_status.free();
+  if (s->arena_base) hipFree(s->arena_base);
+  hipStreamDestroy(s->compute);
+  hipStreamDestroy(s->copy);
+  delete s;
+}

@@ -162,6 +162,32 @@ int fpsx_fps_batch(const float *points, size_t dim, const uint64_t *offsets, siz
                    const uint64_t *n_samples, const uint64_t *start_idx, uint64_t *out);
 
 /*
+ * Serving loop: requests (one 3-D cloud each) are submitted from any thread
+ * and sampled by a scheduler thread that
+ *   - admits pending requests, oldest first, while their device footprint
+ *     (points + min-distances + picks) fits `hbm_budget_bytes` of HBM (one
+ *     arena allocated at creation) and their workgroups fit the GPU;
+ *   - advances every admitted request by up to `chunk_passes` passes per
+ *     launch of the batch kernel (continuous batching: requests that arrive
+ *     while others run join at the next launch; uploads overlap the kernel);
+ *   - retires finished requests and wakes their waiters.
+ * Each request's picks equal fpsx_fps_sampling on its cloud alone.
+ * block/ppt select the batch-kernel shape (0, 0: 1024 x 16).
+ * fpsx_server_submit returns FPSX_EBUDGET for a request larger than the whole
+ * budget.  fpsx_server_wait blocks until the request is done, copies its
+ * picks and its latency (submit to completion, seconds).  Destroy drains.
+ */
+typedef struct fpsx_server fpsx_server;
+int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passes, int block, int ppt,
+                       fpsx_server **out);
+int fpsx_server_submit(fpsx_server *s, const float *points, size_t n_points, size_t dim,
+                       size_t n_samples, uint64_t start_idx, uint64_t *request_id);
+int fpsx_server_wait(fpsx_server *s, uint64_t request_id, uint64_t *out, double *latency_s);
+int fpsx_server_stats(fpsx_server *s, uint64_t *launches, uint64_t *completed, uint64_t *samples,
+                      double *busy_s, size_t *hbm_in_use, size_t *pending, size_t *active);
+void fpsx_server_destroy(fpsx_server *s);
+
+/*
  * Bucket FPS entry with the exact signature and return codes of the
  * reference's C API bucket_fps_kdline (src/wrapper.hpp:118-132).
  * Same picks as the reference: the KD-line tree of height `height` is built on

@@ -1,0 +1,73 @@
+"""GPU parity of the serving loop: requests of mixed sizes submitted from
+several threads, admitted by HBM budget and batched continuously, each equal
+to the oracle's fps_sampling of its cloud alone."""
+import threading
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def requests(seed, k, lo, hi):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        pc = rng.random((n, 3)).astype(np.float32) if i % 4 else rng.integers(0, 5, (n, 3)).astype(np.float32)
+        m = max(1, int(n * rng.choice([1 / 8, 1 / 4, 1 / 2])))
+        out.append((pc, m, int(rng.integers(0, n))))
+    return out
+
+
+def test_serve_mixed_from_threads(gpu):
+    from fpsample_amd.serve import Server
+
+    reqs = requests(7, 24, 64, 40000)
+    results = {}
+    with Server(hbm_budget_bytes=256 << 20, chunk_passes=128) as srv:
+        def client(ids):
+            for i in ids:
+                pc, m, s = reqs[i]
+                rid = srv.submit(pc, m, s)
+                results[i] = srv.wait(rid)
+
+        ts = [threading.Thread(target=client, args=(list(range(t, len(reqs), 4)),)) for t in range(4)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["error"] == 0
+    for i, (pc, m, s) in enumerate(reqs):
+        np.testing.assert_array_equal(results[i][0], O.fps(pc, m, s), err_msg=f"request {i}")
+
+
+def test_serve_tight_budget_queues(gpu):
+    from fpsample_amd.serve import BudgetError, Server
+
+    reqs = requests(8, 10, 20000, 60000)
+    # room for about one request at a time: the rest wait for memory
+    with Server(hbm_budget_bytes=2 << 20, chunk_passes=64) as srv:
+        ids = [srv.submit(pc, m, s) for pc, m, s in reqs]
+        outs = [srv.wait(r)[0] for r in ids]
+        with pytest.raises(BudgetError):
+            srv.submit(np.zeros((300000, 3), np.float32), 10, 0)
+    for (pc, m, s), got in zip(reqs, outs):
+        np.testing.assert_array_equal(got, O.fps(pc, m, s))
+
+
+def test_serve_same_alone_and_in_mix(gpu):
+    from fpsample_amd.serve import Server
+
+    reqs = requests(9, 12, 500, 30000)
+    with Server(hbm_budget_bytes=512 << 20, chunk_passes=32) as srv:
+        alone = []
+        for pc, m, s in reqs:
+            alone.append(srv.wait(srv.submit(pc, m, s))[0])
+        ids = [srv.submit(pc, m, s) for pc, m, s in reqs]
+        mixed = [srv.wait(r)[0] for r in ids]
+    for a, b in zip(alone, mixed):
+        np.testing.assert_array_equal(a, b)
