@@ -268,12 +268,39 @@ def run_serve(args) -> dict:
     }
 
 
+def run_copy(args) -> dict:
+    """Calibration: device-to-device copies of a known 256 MiB (the runtime's
+    plain copy kernel), for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+
+    from fpsample_amd import device as D
+
+    nbytes = 256 << 20
+    src = D.DeviceArray((nbytes // 4,), np.float32)
+    dst = D.DeviceArray((nbytes // 4,), np.float32)
+    h = D.hip()
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    for _ in range(args.warmup):
+        D._check(h.hipMemcpy(dst.ptr, src.ptr, nbytes, 3), "hipMemcpy D2D")
+    D.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        D._check(h.hipMemcpy(dst.ptr, src.ptr, nbytes, 3), "hipMemcpy D2D")
+    D.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"metric": "device copy GB/s (read + write)", "value": round(2 * nbytes / dt / 1e9, 2),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": "hipMemcpy device-to-device, 268435456 bytes per copy"}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="gauss1m", choices=sorted(WORKLOADS) + ["batch256", "serve"])
+    ap.add_argument("--workload", default="gauss1m", choices=sorted(WORKLOADS) + ["batch256", "serve", "copy256m"])
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--autotune", default="on", choices=["on", "off"])
@@ -290,6 +317,9 @@ def main() -> int:
         return 0
     if args.workload == "serve":
         print(json.dumps(run_serve(args)), flush=True)
+        return 0
+    if args.workload == "copy256m":
+        print(json.dumps(run_copy(args)), flush=True)
         return 0
 
     from fpsample_amd import device as D
