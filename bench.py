@@ -141,18 +141,25 @@ def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float) -> dict:
     }
 
 
-def _read_traffic(workload: str):
-    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+def _read_traffic(workload: str, plan: dict):
+    """HBM bytes per launch of the FPS kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known
+    size), when that summary was taken with the plan this run uses."""
     path = os.path.join(ROOT, "profiles", f"fps_kernel_{workload}_traffic.txt")
     if not os.path.exists(path):
         return None
+    kv = {}
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch="):
-            try:
-                return float(line.split("=", 1)[1])
-            except ValueError:
-                return None
-    return None
+        if "=" in line and not line.startswith("#"):
+            k, v = line.strip().split("=", 1)
+            kv[k] = v
+    want = f"{plan.get('kind')}:{plan.get('block')}:{plan.get('ppt')}:{plan.get('grid')}"
+    if kv.get("plan") != want:
+        return None
+    try:
+        return float(kv["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def run_batch(args) -> dict:
@@ -386,7 +393,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _read_traffic(args.workload),
+            "traffic": _read_traffic(args.workload, plan),
             "kernel_ms": round(kernel_s * 1e3, 4),
             "algorithmic_bytes_per_launch": alg_bytes,
         },

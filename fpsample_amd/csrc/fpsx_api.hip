@@ -519,7 +519,7 @@ FPSX_EXPORT int fpsx_autotune(const float *d_points, size_t stride_point, size_t
   Ctx &c = *cp;
   std::lock_guard<std::mutex> lk(c.mu);
   hipStream_t st = stream ? (hipStream_t)stream : c.stream;
-  const size_t m = std::max<size_t>(2, std::min(passes ? passes : 64, n_points));
+  const size_t m = std::max<size_t>(2, std::min(passes ? passes : 256, n_points));
   HIP_TRY(c.starts.reserve(sizeof(uint64_t)));
   HIP_TRY(c.out.reserve(m * sizeof(uint64_t)));
   launch_fill_u64((uint64_t *)c.starts.p, 1, 0, st);
@@ -528,9 +528,9 @@ FPSX_EXPORT int fpsx_autotune(const float *d_points, size_t stride_point, size_t
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  double best = 1e30;
-  const Plan *bp = nullptr;
-  for (const Plan &q : cands) {
+  std::vector<double> times(cands.size(), 1e30);
+  for (size_t ci = 0; ci < cands.size(); ++ci) {
+    const Plan &q = cands[ci];
     float tmin = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {
       HIP_TRY(hipEventRecord(e0, st));
@@ -546,13 +546,18 @@ FPSX_EXPORT int fpsx_autotune(const float *d_points, size_t stride_point, size_t
     HIP_TRY(hipMemcpyAsync(c.host_status, c.status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (*c.host_status != 0) continue;  // a candidate that failed is never chosen
-    if (tmin < best) {
-      best = tmin;
-      bp = &q;
-    }
+    times[ci] = tmin;
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  // Within 3% of the fastest, prefer the fewest workgroups: the choice is
+  // then stable from run to run, and fewer workgroups exchange less.
+  const double best = *std::min_element(times.begin(), times.end());
+  const Plan *bp = nullptr;
+  for (size_t ci = 0; ci < cands.size(); ++ci) {
+    if (times[ci] > best * 1.03) continue;
+    if (!bp || cands[ci].grid < bp->grid) bp = &cands[ci];
+  }
   if (!bp) return fail(FPSX_EHIP, "every candidate plan failed");
   c.tuned[std::make_tuple(n_points, dim, tie)] = std::make_pair(bp->k, bp->grid);
   if (chosen_block) *chosen_block = (int)bp->block;

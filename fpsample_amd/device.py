@@ -198,7 +198,7 @@ def fps_sampling_tensor(points, n_samples: int, start_idx: Union[int, Sequence[i
 
 
 def autotune(points, stride_point: int, stride_dim: int, n_points: int, dim: int,
-             tie: int = TIE_HIGHEST, passes: int = 64, stream: Optional[int] = None) -> dict:
+             tie: int = TIE_HIGHEST, passes: int = 256, stream: Optional[int] = None) -> dict:
     """Time every feasible kernel shape on this cloud and keep the fastest for
     clouds of the same (n_points, dim, tie); returns the chosen shape."""
     b, p, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

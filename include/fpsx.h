@@ -67,9 +67,10 @@ int fpsx_plan_info(size_t n_points, size_t dim, int tie, int *kind, int *block, 
 
 /*
  * Autotuner: times every feasible kernel shape (threads per workgroup, points
- * per thread, workgroups) for `passes` passes (0: 64) on the given device
- * cloud and keeps the fastest for clouds of exactly this (n_points, dim,
- * tie) on the current device; later calls plan with it.  The chosen shape is
+ * per thread, workgroups) for `passes` passes (0: 256) on the given device
+ * cloud and keeps, among the shapes within 3% of the fastest, the one with
+ * the fewest workgroups, for clouds of exactly this (n_points, dim, tie) on
+ * the current device; later calls plan with it.  The chosen shape is
  * returned through the optional pointers.  Without tuning the planner uses a
  * per-pass time model.
  */
