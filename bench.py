@@ -4,8 +4,9 @@
 A step is one vanilla FPS run (fps_sampling semantics, src/lib.cpp:188-246)
 over one synthetic cloud already resident in HBM: one call of
 fpsx_fps_sampling_device, i.e. the exchange-area clear and the single
-persistent FPS launch that makes all M-1 passes.  Before the warmup the
-library's autotuner (fpsx_autotune, untimed) picks the kernel shape.
+persistent FPS launch that makes all M-1 passes.  The kernel shape comes
+from the library's planner (per-pass time model); --autotune on times every
+shape first (untimed) instead.
 
 Default workload (the metric's configuration): one cloud of N = 1,048,576
 points, D = 3 float32, a mixture of 64 isotropic Gaussian clusters (sigma
@@ -310,7 +311,9 @@ def main() -> int:
     ap.add_argument("--workload", default="gauss1m", choices=sorted(WORKLOADS) + ["batch256", "serve", "copy256m"])
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--autotune", default="on", choices=["on", "off"])
+    ap.add_argument("--autotune", default="off", choices=["on", "off"],
+                    help="time every kernel shape on this cloud first (off: the planner's "
+                         "per-pass time model, which picks the same shape for gauss1m)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
