@@ -344,7 +344,7 @@ def main() -> int:
     d_out = D.DeviceArray((m,), np.uint64)
     if args.autotune == "on":
         D.autotune(d_pts, 1, n, n, 3)  # untimed: picks the kernel shape for this cloud
-    plan = D.plan_info(n, 3)
+    plan = D.plan_info(n, 3, n_samples=m)
 
     def step():
         D.fps_sampling_device(d_pts, n, 3, m, d_start, 1, d_out)

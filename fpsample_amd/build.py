@@ -36,8 +36,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", CSRC, "-I", INCLUDE,
 ]
 
-SOURCES_HIP = ["fps_kernels.hip", "fps_batch.hip", "fpsx_api.hip", "fpsx_batch.hip",
-               "fpsx_server.cpp"]
+SOURCES_HIP = ["fps_kernels.hip", "fps_batch.hip", "fps_pruned.hip", "fpsx_api.hip",
+               "fpsx_batch.hip", "fpsx_server.cpp", "fpsx_spatial.hip"]
 SOURCES_HOST = ["kdline.cpp"]
 
 

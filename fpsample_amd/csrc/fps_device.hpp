@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fps_kernels.hpp"
+
 namespace fpsx {
 
 constexpr int kTieHighest = 0;
@@ -238,6 +240,45 @@ __device__ __forceinline__ bool exchange(uint64_t *base, uint32_t R, uint32_t pi
 #pragma unroll
   for (int k = 0; k < NW; ++k) win[k] = xw[k][wl];
   return ok_all;
+}
+
+// Records pick s: lane 0 of the calling wave writes the index, lanes j < D the
+// coordinates when the caller asked for them (the fused `pc[idx]` gather).
+template <int D>
+__device__ __forceinline__ void record_pick(const FpsParams &a, uint32_t s, uint32_t pidx,
+                                            const float (&pv)[D]) {
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) a.out[s] = pidx;
+  if (a.out_pts != nullptr && lane < D) {
+    float v = pv[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j) v = (lane == j) ? pv[j] : v;
+    a.out_pts[(size_t)s * D + lane] = v;
+  }
+}
+
+// Same for the any-dimension kernel, coordinates read from LDS.
+__device__ __forceinline__ void record_pick_dyn(const FpsParams &a, uint32_t s, uint32_t pidx,
+                                                const float *pv, uint32_t dim) {
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) a.out[s] = pidx;
+  if (a.out_pts != nullptr)
+    for (uint32_t j = lane; j < dim; j += 64) a.out_pts[(size_t)s * dim + j] = pv[j];
+}
+
+// Coordinates of a forced pick (multi-start prefix).  An index outside the
+// cloud is reported through the status word and replaced by 0, so no access
+// ever leaves the arrays.
+template <int D>
+__device__ __forceinline__ uint32_t load_forced(const FpsParams &a, uint32_t s, float (&pv)[D]) {
+  uint64_t idx = a.starts[s];
+  if (idx >= a.n) {
+    if (threadIdx.x == 0) atomicOr(a.status, kStatusBadStart);
+    idx = 0;
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) pv[j] = a.pts[(size_t)idx * a.sp + (size_t)j * a.ld];
+  return (uint32_t)idx;
 }
 
 }  // namespace fpsx

@@ -1,0 +1,31 @@
+// fps_pruned.hpp -- host-visible description of the pruned FPS kernel and of
+// the spatial ordering it needs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fps_kernels.hpp"
+
+namespace fpsx {
+
+struct PrunedParams {
+  FpsParams fps;          // the cloud (original order), picks, exchange, status
+  const float *sorted;    // SoA coordinates in spatial order: sorted[j * sorted_ld + pos]
+  uint64_t sorted_ld;
+  const uint32_t *perm;   // original index of every position of the spatial order
+};
+
+const KernelEntry *pruned_table(size_t *count);
+
+// Device scratch of the spatial ordering (grow-only, owned by the caller).
+struct SpatialScratch;
+
+// Enqueue on `st`: the spatial order of the n points of `pts` (coordinate j of
+// point i at pts[i*sp + j*ld], dim == 3), grouped in blocks of `block` positions
+// whose members are sorted by original index; writes sorted (SoA, stride n)
+// and perm.  Returns a FPSX_* code.
+int spatial_order(const float *pts, uint64_t sp, uint64_t ld, uint32_t n, uint32_t block,
+                  float *sorted, uint32_t *perm, SpatialScratch &scratch, hipStream_t st);
+
+}  // namespace fpsx

@@ -22,6 +22,7 @@
 
 #include "fps_device.hpp"
 #include "fps_kernels.hpp"
+#include "fps_pruned.hpp"
 #include "fpsx.h"
 #include "fpsx_internal.hpp"
 #include "kdline.hpp"
@@ -135,6 +136,14 @@ static const KernelEntry *find_resident(int dim, int B, int ppt, int tie) {
   return nullptr;
 }
 
+static const KernelEntry *find_pruned(int dim, int B, int ppt, int tie) {
+  size_t cnt;
+  const KernelEntry *t = pruned_table(&cnt);
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].dim == dim && t[i].block == B && t[i].ppt == ppt && t[i].tie == tie) return &t[i];
+  return nullptr;
+}
+
 static const KernelEntry *find_stream(int dim, int B, int tie) {
   size_t cnt;
   const KernelEntry *t = stream_table(&cnt);
@@ -166,13 +175,14 @@ static int plan_stream(Ctx &c, size_t n, size_t dim, int tie, int B, size_t want
   return FPSX_OK;
 }
 
-static int plan_resident_shape(Ctx &c, size_t n, const KernelEntry *k, Plan &p) {
+static int plan_resident_shape(Ctx &c, size_t n, const KernelEntry *k, Plan &p,
+                               PlanKind kind = PlanKind::Resident) {
   const size_t per = (size_t)k->block * k->ppt;
   const size_t g = (n + per - 1) / per;
   if (g > 1 && g > (size_t)c.max_blocks(k, 0)) return -1;
   if (g > (size_t)k->block) return -1;  // exchange polls one workgroup per lane
   if (g > (size_t)kPageWords) return -1;
-  p.kind = PlanKind::Resident;
+  p.kind = kind;
   p.k = k;
   p.block = (uint32_t)k->block;
   p.grid = (uint32_t)g;
@@ -198,11 +208,19 @@ static int plan_from_tuned(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
   return plan_stream(c, n, dim, tie, k->block, it->second.second, k->dim == 0, p);
 }
 
-int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
+int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
   const char *env = std::getenv("FPSX_PLAN");
   if (env && *env) {
     std::string s(env);
     int a = 0, b = 0, g = 0;
+    if (std::sscanf(s.c_str(), "pruned:%d:%d", &a, &b) == 2) {
+      const KernelEntry *k = find_pruned((int)dim, a, b, tie);
+      if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no pruned kernel for dim " +
+                                           std::to_string(dim) + " shape " + s);
+      if (plan_resident_shape(c, n, k, p, PlanKind::Pruned) != FPSX_OK)
+        return fail(FPSX_ETOOBIG, "FPSX_PLAN: cloud exceeds the co-resident grid of " + s);
+      return FPSX_OK;
+    }
     if (std::sscanf(s.c_str(), "resident:%d:%d", &a, &b) == 2) {
       const KernelEntry *k = find_resident((int)dim, a, b, tie);
       if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no resident kernel for dim " +
@@ -221,6 +239,19 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
   }
 
   if (plan_from_tuned(c, n, dim, tie, p) == FPSX_OK) return FPSX_OK;
+
+  // Large 3-D clouds with many samples: spatial order + per-wave pruning.
+  // The ordering costs a few radix-sort passes over the cloud, repaid once
+  // most passes touch few waves; the shape holding the most points per
+  // workgroup keeps the exchange small.
+  const char *np = std::getenv("FPSX_NO_PRUNE");
+  if (dim == 3 && n >= 65536 && m >= 1024 && !(np && np[0] == '1')) {
+    static const int kPr[][2] = {{1024, 16}, {256, 32}, {1024, 8}, {1024, 4}};
+    for (auto &sh : kPr) {
+      const KernelEntry *k = find_pruned(3, sh[0], sh[1], tie);
+      if (k && plan_resident_shape(c, n, k, p, PlanKind::Pruned) == FPSX_OK) return FPSX_OK;
+    }
+  }
 
   // Single workgroup: the smallest resident shape that holds the cloud.
   static const int kSmall[][2] = {{256, 1}, {256, 2}, {256, 4}, {1024, 1}, {1024, 2},
@@ -324,7 +355,21 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
     a.dmin = (float *)c.dmin.p;
     launch_fill_f32(a.dmin, n, init, st);
   }
+  PrunedParams pa;
   void *args[] = {&a};
+  if (p.kind == PlanKind::Pruned) {
+    HIP_TRY(c.sorted.reserve(n * 3 * sizeof(float)));
+    HIP_TRY(c.perm.reserve(n * sizeof(uint32_t)));
+    if (!c.spatial) c.spatial = new_spatial_scratch();
+    int rc = spatial_order(d_pts, sp, ld, (uint32_t)n, (uint32_t)(64 * p.k->ppt),
+                           (float *)c.sorted.p, (uint32_t *)c.perm.p, *c.spatial, st);
+    if (rc) return rc;
+    pa.fps = a;
+    pa.sorted = (const float *)c.sorted.p;
+    pa.sorted_ld = n;
+    pa.perm = (const uint32_t *)c.perm.p;
+    args[0] = &pa;
+  }
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
   if (c.timing) {
     if (c.events_used == c.events.size()) {
@@ -371,7 +416,7 @@ int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
   Ctx &c = *cp;
   std::lock_guard<std::mutex> lk(c.mu);
   Plan p;
-  rc = plan(c, n, dim, tie, p);
+  rc = plan(c, n, dim, tie, m, p);
   if (rc) return rc;
   hipStream_t st = c.stream;
   const size_t bytes = n * dim * sizeof(float);
@@ -456,7 +501,7 @@ FPSX_EXPORT int fpsx_fps_sampling_device_strided(const float *d_points, size_t s
   Ctx &c = *cp;
   std::lock_guard<std::mutex> lk(c.mu);
   Plan p;
-  rc = plan(c, n_points, dim, tie, p);
+  rc = plan(c, n_points, dim, tie, n_samples, p);
   if (rc) return rc;
   hipStream_t st = stream ? (hipStream_t)stream : c.stream;
   return launch_fps(c, p, d_points, stride_point, stride_dim, n_points, dim, n_samples, d_start,
@@ -489,8 +534,8 @@ FPSX_EXPORT int fpsx_device_status(void *stream) {
   return status_code(*c.host_status);
 }
 
-FPSX_EXPORT int fpsx_plan_info(size_t n_points, size_t dim, int tie, int *kind, int *block,
-                               int *ppt, int *grid, int *chunk) {
+FPSX_EXPORT int fpsx_plan_info(size_t n_points, size_t dim, int tie, size_t n_samples, int *kind,
+                               int *block, int *ppt, int *grid, int *chunk) {
   t_err.clear();
   Ctx *cp = nullptr;
   int rc = get_ctx(&cp);
@@ -498,7 +543,7 @@ FPSX_EXPORT int fpsx_plan_info(size_t n_points, size_t dim, int tie, int *kind, 
   Ctx &c = *cp;
   std::lock_guard<std::mutex> lk(c.mu);
   Plan p;
-  rc = plan(c, n_points, dim, tie, p);
+  rc = plan(c, n_points, dim, tie, n_samples, p);
   if (rc) return rc;
   if (kind) *kind = (int)p.kind;
   if (block) *block = (int)p.block;

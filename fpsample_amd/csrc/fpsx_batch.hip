@@ -309,7 +309,7 @@ FPSX_EXPORT int fpsx_batch_run(fpsx_batch *b, void *stream) {
     for (size_t i = 0; i < b->specs.size(); ++i) {
       const JobSpec &s = b->specs[i];
       Plan p;
-      int rc = plan(c, s.n, b->dim, kTieHighest, p);
+      int rc = plan(c, s.n, b->dim, kTieHighest, s.m, p);
       if (rc) return rc;
       rc = launch_fps(c, p, b->d_pts + s.pt_off, 1, b->ld, s.n, b->dim, s.m,
                       (const uint64_t *)c.starts.p + i, 1, b->d_out + s.out_off, nullptr,

@@ -37,7 +37,8 @@ struct Ctx {
   int backoff = 0;   // poll backoff rounds (FPSX_XCHG_BACKOFF)
   uint32_t *host_status = nullptr;  // pinned
   std::mutex mu;
-  DevBuf pts, rows, starts, out, xchg, status, dmin;
+  DevBuf pts, rows, starts, out, xchg, status, dmin, sorted, perm;
+  struct SpatialScratch *spatial = nullptr;
   std::map<std::pair<const void *, size_t>, int> occ;  // co-resident workgroups per kernel
   int max_blocks(const KernelEntry *k, size_t shmem);
   // Autotuned plans: (n_points, dim, tie) -> (kernel entry, grid).
@@ -48,7 +49,7 @@ struct Ctx {
   size_t events_used = 0;
 };
 
-enum class PlanKind : int { Resident = 0, Stream = 1 };
+enum class PlanKind : int { Resident = 0, Stream = 1, Pruned = 2 };
 
 struct Plan {
   PlanKind kind = PlanKind::Resident;
@@ -61,13 +62,15 @@ struct Plan {
 
 int fail(int code, const std::string &msg);
 int get_ctx(Ctx **out);
-int plan(Ctx &c, size_t n, size_t dim, int tie, Plan &p);
+// Kernel and grid for a cloud of n points of dimension dim and m samples.
+int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p);
 // Enqueue one FPS run on device data: coordinate j of point i at
 // d_pts[i*sp + j*ld].  d_out_pts (may be null) receives the picks' coordinates.
 int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, size_t n,
                size_t dim, size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out,
                float *d_out_pts, float init, hipStream_t st);
 int status_code(uint32_t st);
+struct SpatialScratch *new_spatial_scratch();
 int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
              const uint64_t *starts, size_t n_starts, uint64_t *out, int tie, float init);
 

@@ -69,7 +69,7 @@ def lib() -> ctypes.CDLL:
         L.fpsx_timing.argtypes = [ctypes.c_int]
         L.fpsx_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
         ip = ctypes.POINTER(ctypes.c_int)
-        L.fpsx_plan_info.argtypes = [sz, sz, ctypes.c_int, ip, ip, ip, ip, ip]
+        L.fpsx_plan_info.argtypes = [sz, sz, ctypes.c_int, sz, ip, ip, ip, ip, ip]
         _fpsx = L
     return _fpsx
 
@@ -229,9 +229,11 @@ def timing_read():
     return ms.value, n.value
 
 
-def plan_info(n_points: int, dim: int, tie: int = TIE_HIGHEST) -> dict:
+def plan_info(n_points: int, dim: int, tie: int = TIE_HIGHEST, n_samples: int = 0) -> dict:
+    """The launch the planner picks (n_samples 0: as for a short run)."""
     v = [ctypes.c_int() for _ in range(5)]
-    _raise(lib().fpsx_plan_info(n_points, dim, tie, *[ctypes.byref(x) for x in v]),
+    _raise(lib().fpsx_plan_info(n_points, dim, tie, n_samples, *[ctypes.byref(x) for x in v]),
            "fpsx_plan_info")
     kind, block, ppt, grid, chunk = (x.value for x in v)
-    return dict(kind=("resident", "stream")[kind], block=block, ppt=ppt, grid=grid, chunk=chunk)
+    return dict(kind=("resident", "stream", "pruned")[kind], block=block, ppt=ppt, grid=grid,
+                chunk=chunk)

@@ -58,12 +58,13 @@ int fpsx_device_count(void);
 int fpsx_device_status(void *stream);
 
 /*
- * The launch the planner picks for a cloud of n_points x dim under `tie`:
- * kind 0 = register-resident, 1 = HBM-streaming; threads per workgroup,
- * points per thread (resident), workgroups, points per workgroup.
+ * The launch the planner picks for a cloud of n_points x dim and n_samples
+ * picks under `tie`: kind 0 = register-resident, 1 = HBM-streaming,
+ * 2 = register-resident with spatial order and per-wave pruning; threads per
+ * workgroup, points per thread, workgroups, points per workgroup.
  */
-int fpsx_plan_info(size_t n_points, size_t dim, int tie, int *kind, int *block, int *ppt,
-                   int *grid, int *chunk);
+int fpsx_plan_info(size_t n_points, size_t dim, int tie, size_t n_samples, int *kind, int *block,
+                   int *ppt, int *grid, int *chunk);
 
 /*
  * Autotuner: times every feasible kernel shape (threads per workgroup, points

@@ -85,9 +85,9 @@ class CApi:
             raise RuntimeError(f"fpsx_kdline_permutation -> {nl}: {self.last_error()}")
         return perm, lo[:nl], hi[:nl]
 
-    def plan_info(self, n, d, tie=HIGHEST):
+    def plan_info(self, n, d, tie=HIGHEST, m=0):
         v = [ctypes.c_int() for _ in range(5)]
-        rc = self.L.fpsx_plan_info(n, d, tie, *[ctypes.byref(x) for x in v])
+        rc = self.L.fpsx_plan_info(n, d, tie, m, *[ctypes.byref(x) for x in v])
         if rc != 0:
             raise RuntimeError(self.last_error())
         kind, block, ppt, grid, chunk = (x.value for x in v)

@@ -224,3 +224,48 @@ def test_device_strided_rows_and_fused_gather(gpu, plan_env):
         want2 = O.fps(pc, 400, [17, 3])
         np.testing.assert_array_equal(out.numpy(), want2)
         np.testing.assert_array_equal(opts.numpy(), pc[want2.astype(np.int64)])
+
+
+PRUNED_SHAPES = ["pruned:1024:16", "pruned:1024:8", "pruned:1024:4", "pruned:256:16", "pruned:256:32"]
+
+
+@pytest.mark.parametrize("shape", PRUNED_SHAPES)
+def test_pruned_shapes(gpu, plan_env, shape):
+    # Spatial order + per-wave box pruning must not change a single pick.
+    plan_env(shape)
+    rng = np.random.default_rng(31)
+    clouds = [("gauss", c2_cloud(40000, seed=3)),
+              ("lattice_dups", np.repeat(rng.integers(0, 6, (6000, 3)), 3, axis=0).astype(np.float32)),
+              ("uniform", rng.random((30000, 3)).astype(np.float32))]
+    for name, pc in clouds:
+        for starts in (0, [len(pc) - 1, 7, 11]):
+            log(f"{shape} {name} starts={starts}")
+            want = O.fps(pc, 2000, starts)
+            np.testing.assert_array_equal(run_fps(gpu, pc, 2000, starts), want,
+                                          err_msg=f"{shape} {name}")
+
+
+def test_pruned_special_values(gpu, plan_env):
+    plan_env("pruned:1024:4")
+    rng = np.random.default_rng(32)
+    pc = (rng.random((9000, 3)) * 4e19).astype(np.float32)
+    pc[[5, 700, 4000]] = np.nan
+    pc[9] = np.inf
+    pc[10, 1] = -np.inf
+    np.testing.assert_array_equal(run_fps(gpu, pc, 600, 0), O.fps(pc, 600, 0))
+    tiny = (rng.random((9000, 3)) * 1e-21).astype(np.float32)
+    np.testing.assert_array_equal(run_fps(gpu, tiny, 600, 0), O.fps(tiny, 600, 0))
+
+
+def test_pruned_auto_plan_and_bucket_order(gpu):
+    rng = np.random.default_rng(33)
+    pc = c2_cloud(120000, seed=5)
+    info = gpu.plan_info(len(pc), 3, m=2048)
+    log(f"auto plan {info}")
+    assert info["kind"] == 2
+    np.testing.assert_array_equal(run_fps(gpu, pc, 2048, 77), O.fps(pc, 2048, 77))
+    # bucket entry: lowest-position ties over the KD-line order, pruned plan
+    lat = rng.integers(0, 40, (100000, 3)).astype(np.float32)
+    rc, out = gpu.kdline(lat, 1500, 123, 7)
+    assert rc == 0, gpu.last_error()
+    np.testing.assert_array_equal(out, O.kdline_quickfps(lat, 1500, 7, 123))
