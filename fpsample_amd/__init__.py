@@ -6,13 +6,14 @@
 
 The functions keep the reference's names, signatures, defaults, assertions,
 dtype coercion and the single draw from numpy's global RNG when start_idx is
-None (src/fpsample/__init__.py:19-65, 174-206).  The work runs on the GPU
+None (src/fpsample/__init__.py:19-103, 174-206).  The work runs on the GPU
 through the pybind11 module ``fpsample_amd._fpsample`` and the C ABI of
 ``libfpsx.so``; there is no CPU fallback: importing this package fails when
 the native extension is missing.
 """
 from __future__ import annotations
 
+import warnings
 from typing import List, Optional, Union
 
 import numpy as np
@@ -22,6 +23,7 @@ try:
         __doc__ as _ext_doc,
         __version__,
         _bucket_fps_kdline_sampling,
+        _fps_npdu_sampling,
         _fps_sampling,
         _fps_sampling_batch,
     )
@@ -79,6 +81,46 @@ def fps_sampling(
     pc = np.asfortranarray(pc, dtype=np.float32)
     start_idx = get_start_idx(n_pts, start_idx)
     return _fps_sampling(pc, n_samples, start_idx)
+
+
+def fps_npdu_sampling(
+    pc: np.ndarray,
+    n_samples: int,
+    w: Optional[int] = None,
+    start_idx: Optional[Union[int, List[int]]] = None,
+) -> np.ndarray:
+    """
+    FPS with nearest-point-distance-updating (NPDU), src/fpsample/__init__.py:68-103.
+
+    After the start, each pick updates only the min-distances of the ``w + 1``
+    points around it in index order, so the result depends on the cloud's
+    index order (dimensional locality); the next pick is still the largest
+    min-distance over all points, lowest index on ties.
+
+    Args:
+        pc (np.ndarray): The input point cloud of shape (n_pts, D).
+        n_samples (int): Number of samples.
+        w (int, default=None): Window of the local update; None means ``int(n_pts / n_samples * 16)``.
+        start_idx (int, default=None): The starting index of sampling; random if None.
+    Returns:
+        np.ndarray: The selected indices of shape (n_samples,), dtype uint64.
+    """
+    assert n_samples >= 1, "n_samples should be >= 1"
+    assert pc.ndim == 2
+    n_pts, _ = pc.shape
+    assert n_pts >= n_samples, "n_pts should be >= n_samples"
+    assert (
+        start_idx is None or 0 <= start_idx < n_pts
+    ), "start_idx should be None or 0 <= start_idx < n_pts"
+    if isinstance(start_idx, list):
+        assert len(start_idx) <= n_samples, "len(start_idx) should be <= n_samples"
+    pc = np.ascontiguousarray(pc, dtype=np.float32)
+    w = w or int(n_pts / n_samples * 16)
+    if w >= n_pts - 1:
+        warnings.warn(f"k is too large, set to {n_pts - 1}")
+        w = n_pts - 1
+    start_idx = get_start_idx(n_pts, start_idx)
+    return _fps_npdu_sampling(pc, n_samples, w, start_idx)
 
 
 def bucket_fps_kdline_sampling(
@@ -163,6 +205,7 @@ __all__ = [
     "__doc__",
     "__version__",
     "fps_sampling",
+    "fps_npdu_sampling",
     "bucket_fps_kdline_sampling",
     "fps_sampling_batch",
     "get_start_idx",

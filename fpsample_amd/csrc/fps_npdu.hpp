@@ -1,0 +1,20 @@
+// fps_npdu.hpp -- host-visible description of the windowed-update (NPDU) FPS kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fps_kernels.hpp"
+
+namespace fpsx {
+
+struct NpduParams {
+  FpsParams fps;    // the cloud, the single start (starts[0]), picks, exchange, status
+  uint64_t window;  // update window w: pick p updates indices p - w/2 .. p + w/2 (shifted into range)
+};
+
+// Entries: dim 3 (coordinates in registers) and dim 0 (any dimension,
+// coordinates read from memory for the window only); tie is always lowest.
+const KernelEntry *npdu_table(size_t *count);
+
+}  // namespace fpsx

@@ -22,6 +22,7 @@
 
 #include "fps_device.hpp"
 #include "fps_kernels.hpp"
+#include "fps_npdu.hpp"
 #include "fps_pruned.hpp"
 #include "fpsx.h"
 #include "fpsx_internal.hpp"
@@ -285,6 +286,52 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
   return plan_stream(c, n, dim, tie, 1024, 0, false, p);
 }
 
+static const KernelEntry *find_npdu(int dim, int B, int ppt) {
+  size_t cnt;
+  const KernelEntry *t = npdu_table(&cnt);
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].dim == dim && t[i].block == B && t[i].ppt == ppt) return &t[i];
+  return nullptr;
+}
+
+// Windowed-update plans.  FPSX_NPDU_PLAN="D:B:PPT" (D = 3 or 0, the
+// any-dimension kernel) overrides.  Otherwise, 3-D clouds keep their
+// coordinates in registers: one workgroup up to 16384 points, else 16384
+// points per workgroup; other dimensions use the any-dimension kernel with
+// the smallest share that holds the cloud in one workgroup, else 32768
+// points per workgroup.
+int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p) {
+  const char *env = std::getenv("FPSX_NPDU_PLAN");
+  if (env && *env) {
+    int d = 0, b = 0, q = 0;
+    if (std::sscanf(env, "%d:%d:%d", &d, &b, &q) != 3 || (d != 0 && d != (int)dim))
+      return fail(FPSX_EINVAL, std::string("FPSX_NPDU_PLAN: bad value ") + env);
+    const KernelEntry *k = find_npdu(d, b, q);
+    if (!k) return fail(FPSX_EINVAL, std::string("FPSX_NPDU_PLAN: no kernel ") + env);
+    if (plan_resident_shape(c, n, k, p, PlanKind::Npdu) != FPSX_OK)
+      return fail(FPSX_ETOOBIG, std::string("FPSX_NPDU_PLAN: cloud exceeds the grid of ") + env);
+    return FPSX_OK;
+  }
+  static const int k3[][2] = {{256, 1}, {256, 2}, {256, 4}, {1024, 4}, {1024, 8}, {1024, 16}};
+  static const int k0[][2] = {{256, 1}, {256, 4}, {1024, 4}, {1024, 16}, {1024, 32}};
+  const bool d3 = dim == 3;
+  const int(*shapes)[2] = d3 ? k3 : k0;
+  const size_t ns = d3 ? sizeof(k3) / sizeof(k3[0]) : sizeof(k0) / sizeof(k0[0]);
+  for (size_t i = 0; i < ns; ++i) {
+    if ((size_t)shapes[i][0] * shapes[i][1] < n) continue;
+    const KernelEntry *k = find_npdu(d3 ? 3 : 0, shapes[i][0], shapes[i][1]);
+    if (k && plan_resident_shape(c, n, k, p, PlanKind::Npdu) == FPSX_OK) return FPSX_OK;
+  }
+  const KernelEntry *k = find_npdu(d3 ? 3 : 0, 1024, d3 ? 16 : 32);
+  if (k && plan_resident_shape(c, n, k, p, PlanKind::Npdu) == FPSX_OK) return FPSX_OK;
+  if (d3) {
+    k = find_npdu(0, 1024, 32);
+    if (k && plan_resident_shape(c, n, k, p, PlanKind::Npdu) == FPSX_OK) return FPSX_OK;
+  }
+  return fail(FPSX_ETOOBIG, "cloud of " + std::to_string(n) +
+                                " points exceeds the windowed-update kernel's co-resident grid");
+}
+
 // Candidate plans the tuner times for a cloud: every feasible resident shape
 // of this dimension plus the streaming kernel at two block sizes.
 static std::vector<Plan> candidates(Ctx &c, size_t n, size_t dim, int tie) {
@@ -306,17 +353,12 @@ static std::vector<Plan> candidates(Ctx &c, size_t n, size_t dim, int tie) {
 // ---------------------------------------------------------------------------
 // Launch on device-resident data.  Caller holds c.mu.
 // ---------------------------------------------------------------------------
-int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, size_t n,
-               size_t dim, size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out,
-               float *d_out_pts, float init, hipStream_t st) {
-  if (p.kind == PlanKind::Stream && sp != 1) {
-    // The streaming kernel re-reads every point on every pass: give it SoA.
-    HIP_TRY(c.pts.reserve(n * dim * sizeof(float)));
-    launch_strided_to_soa(d_pts, sp, ld, (float *)c.pts.p, (uint32_t)n, (uint32_t)dim, n, st);
-    d_pts = (const float *)c.pts.p;
-    sp = 1;
-    ld = n;
-  }
+// Kernel arguments common to every FPS kernel; clears the exchange area and
+// the status word on `st`.
+static int common_params(Ctx &c, const float *d_pts, size_t sp, size_t ld, size_t n, size_t dim,
+                         size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out,
+                         float *d_out_pts, float init, uint32_t chunk, hipStream_t st,
+                         FpsParams &a) {
   // Exchange tuning knobs (read per launch so a tuner can vary them).
   const char *rep_env = std::getenv("FPSX_XCHG_REPLICAS");
   const char *bo_env = std::getenv("FPSX_XCHG_BACKOFF");
@@ -331,7 +373,6 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
   HIP_TRY(c.status.reserve(sizeof(uint64_t)));
   launch_fill_u64((uint64_t *)c.xchg.p, xwords, 0, st);
   launch_fill_u64((uint64_t *)c.status.p, 1, 0, st);
-  FpsParams a;
   std::memset(&a, 0, sizeof(a));
   a.pts = d_pts;
   a.ld = ld;
@@ -349,27 +390,13 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
   a.spin_ticks = c.spin_ticks;
   a.replicas = reps;
   a.backoff = (uint32_t)c.backoff;
-  a.chunk = p.chunk;
-  if (p.kind == PlanKind::Stream) {
-    HIP_TRY(c.dmin.reserve(n * sizeof(float)));
-    a.dmin = (float *)c.dmin.p;
-    launch_fill_f32(a.dmin, n, init, st);
-  }
-  PrunedParams pa;
-  void *args[] = {&a};
-  if (p.kind == PlanKind::Pruned) {
-    HIP_TRY(c.sorted.reserve(n * 3 * sizeof(float)));
-    HIP_TRY(c.perm.reserve(n * sizeof(uint32_t)));
-    if (!c.spatial) c.spatial = new_spatial_scratch();
-    int rc = spatial_order(d_pts, sp, ld, (uint32_t)n, (uint32_t)(64 * p.k->ppt),
-                           (float *)c.sorted.p, (uint32_t *)c.perm.p, *c.spatial, st);
-    if (rc) return rc;
-    pa.fps = a;
-    pa.sorted = (const float *)c.sorted.p;
-    pa.sorted_ld = n;
-    pa.perm = (const uint32_t *)c.perm.p;
-    args[0] = &pa;
-  }
+  a.chunk = chunk;
+  return FPSX_OK;
+}
+
+// Enqueue the planned kernel with its argument block (timed when asked).
+static int dispatch(Ctx &c, const Plan &p, void *arg, hipStream_t st) {
+  void *args[] = {arg};
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
   if (c.timing) {
     if (c.events_used == c.events.size()) {
@@ -391,13 +418,65 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
   if (p.grid > 1 && coop) {
     HIP_TRY(hipLaunchCooperativeKernel(p.k->fn, dim3(p.grid), dim3(p.block), args,
                                        (unsigned)p.shmem, st));
-  } else if (p.grid > 1) {
-    HIP_TRY(hipLaunchKernel(p.k->fn, dim3(p.grid), dim3(p.block), args, p.shmem, st));
   } else {
-    HIP_TRY(hipLaunchKernel(p.k->fn, dim3(1), dim3(p.block), args, p.shmem, st));
+    HIP_TRY(hipLaunchKernel(p.k->fn, dim3(p.grid), dim3(p.block), args, p.shmem, st));
   }
   if (ev) HIP_TRY(hipEventRecord(ev->second, st));
   return FPSX_OK;
+}
+
+int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, size_t n,
+               size_t dim, size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out,
+               float *d_out_pts, float init, hipStream_t st) {
+  if (p.kind == PlanKind::Npdu) return fail(FPSX_EINVAL, "launch_fps: windowed-update plan");
+  if (p.kind == PlanKind::Stream && sp != 1) {
+    // The streaming kernel re-reads every point on every pass: give it SoA.
+    HIP_TRY(c.pts.reserve(n * dim * sizeof(float)));
+    launch_strided_to_soa(d_pts, sp, ld, (float *)c.pts.p, (uint32_t)n, (uint32_t)dim, n, st);
+    d_pts = (const float *)c.pts.p;
+    sp = 1;
+    ld = n;
+  }
+  FpsParams a;
+  int rc = common_params(c, d_pts, sp, ld, n, dim, m, d_starts, n_starts, d_out, d_out_pts, init,
+                         p.chunk, st, a);
+  if (rc) return rc;
+  if (p.kind == PlanKind::Stream) {
+    HIP_TRY(c.dmin.reserve(n * sizeof(float)));
+    a.dmin = (float *)c.dmin.p;
+    launch_fill_f32(a.dmin, n, init, st);
+  }
+  PrunedParams pa;
+  void *arg = &a;
+  if (p.kind == PlanKind::Pruned) {
+    HIP_TRY(c.sorted.reserve(n * 3 * sizeof(float)));
+    HIP_TRY(c.perm.reserve(n * sizeof(uint32_t)));
+    if (!c.spatial) c.spatial = new_spatial_scratch();
+    int rc = spatial_order(d_pts, sp, ld, (uint32_t)n, (uint32_t)(64 * p.k->ppt),
+                           (float *)c.sorted.p, (uint32_t *)c.perm.p, *c.spatial, st);
+    if (rc) return rc;
+    pa.fps = a;
+    pa.sorted = (const float *)c.sorted.p;
+    pa.sorted_ld = n;
+    pa.perm = (const uint32_t *)c.perm.p;
+    arg = &pa;
+  }
+  return dispatch(c, p, arg, st);
+}
+
+// Enqueue one windowed-update run (single start d_start[0]).  Caller holds c.mu.
+int launch_npdu(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, size_t n,
+                size_t dim, size_t m, size_t window, const uint64_t *d_start, uint64_t *d_out,
+                float *d_out_pts, hipStream_t st) {
+  if (p.kind != PlanKind::Npdu) return fail(FPSX_EINVAL, "launch_npdu: not a windowed-update plan");
+  NpduParams a;
+  int rc = common_params(c, d_pts, sp, ld, n, dim, m, d_start, 1, d_out, d_out_pts, INFINITY,
+                         p.chunk, st, a.fps);
+  if (rc) return rc;
+  // Any window of 2n or more covers the whole cloud; the clamp keeps the
+  // kernel's index arithmetic far from overflow.
+  a.window = std::min<uint64_t>((uint64_t)window, (uint64_t)4 * n + 4);
+  return dispatch(c, p, &a, st);
 }
 
 int status_code(uint32_t st) {
@@ -518,6 +597,87 @@ FPSX_EXPORT int fpsx_fps_sampling_device(const float *d_points, size_t ld, size_
   }
   return fpsx_fps_sampling_device_strided(d_points, 1, ld, n_points, dim, n_samples, d_start,
                                           n_start, d_out, nullptr, tie, init, stream);
+}
+
+FPSX_EXPORT int fpsx_fps_npdu_sampling_device(const float *d_points, size_t stride_point,
+                                              size_t stride_dim, size_t n_points, size_t dim,
+                                              size_t n_samples, size_t w, const uint64_t *d_start,
+                                              uint64_t *d_out, float *d_out_points, void *stream) {
+  t_err.clear();
+  if (dim == 0) return fail(FPSX_EBADDIM, "dim must be >= 1");
+  if (!d_points || !d_start || !d_out) return fail(FPSX_EINVAL, "null pointer argument");
+  if (stride_point == 0 || (dim > 1 && stride_dim == 0))
+    return fail(FPSX_EINVAL, "zero stride");
+  if (n_points == 0 || n_samples == 0) return fail(FPSX_EINVAL, "empty cloud or no samples");
+  if (n_points >= (size_t(1) << 31) || n_samples >= (size_t(1) << 32))
+    return fail(FPSX_ETOOBIG, "n_points must be < 2^31");
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  Plan p;
+  rc = plan_npdu(c, n_points, dim, p);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+  return launch_npdu(c, p, d_points, stride_point, stride_dim, n_points, dim, n_samples, w,
+                     d_start, d_out, d_out_points, st);
+}
+
+FPSX_EXPORT int fpsx_fps_npdu_sampling(const float *points, size_t n_points, size_t dim,
+                                       size_t n_samples, size_t w, size_t start_idx,
+                                       uint64_t *out) {
+  t_err.clear();
+  if (dim == 0) return fail(FPSX_EBADDIM, "points must have at least one column");
+  if (!points || !out) return fail(FPSX_EINVAL, "null pointer argument");
+  if (n_samples == 0 || n_samples > n_points)
+    return fail(FPSX_EINVAL, "n_samples must be in [1, n_points]: n_samples=" +
+                                 std::to_string(n_samples) + ", P=" + std::to_string(n_points));
+  if (start_idx >= n_points) return fail(FPSX_EBADSTART, "start_idx out of range");
+  if (n_points >= (size_t(1) << 31)) return fail(FPSX_ETOOBIG, "n_points must be < 2^31");
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  Plan p;
+  rc = plan_npdu(c, n_points, dim, p);
+  if (rc) return rc;
+  hipStream_t st = c.stream;
+  const size_t bytes = n_points * dim * sizeof(float);
+  HIP_TRY(c.pts.reserve(bytes));
+  HIP_TRY(c.starts.reserve(sizeof(uint64_t)));
+  HIP_TRY(c.out.reserve(n_samples * sizeof(uint64_t)));
+  // Row-major points as given: the 3-D kernel reads them once into
+  // registers, the any-dimension kernel reads only the windows.
+  HIP_TRY(hipMemcpyAsync(c.pts.p, points, bytes, hipMemcpyHostToDevice, st));
+  const uint64_t s64 = start_idx;
+  HIP_TRY(hipMemcpyAsync(c.starts.p, &s64, sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  rc = launch_npdu(c, p, (const float *)c.pts.p, dim, 1, n_points, dim, n_samples, w,
+                   (const uint64_t *)c.starts.p, (uint64_t *)c.out.p, nullptr, st);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, c.out.p, n_samples * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(c.host_status, c.status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return status_code(*c.host_status);
+}
+
+FPSX_EXPORT int fpsx_npdu_plan_info(size_t n_points, size_t dim, int *kernel_dim, int *block,
+                                    int *ppt, int *grid) {
+  t_err.clear();
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  Plan p;
+  rc = plan_npdu(c, n_points, dim, p);
+  if (rc) return rc;
+  if (kernel_dim) *kernel_dim = p.k->dim;
+  if (block) *block = (int)p.block;
+  if (ppt) *ppt = p.k->ppt;
+  if (grid) *grid = (int)p.grid;
+  return FPSX_OK;
 }
 
 FPSX_EXPORT int fpsx_device_status(void *stream) {

@@ -49,7 +49,7 @@ struct Ctx {
   size_t events_used = 0;
 };
 
-enum class PlanKind : int { Resident = 0, Stream = 1, Pruned = 2 };
+enum class PlanKind : int { Resident = 0, Stream = 1, Pruned = 2, Npdu = 3 };
 
 struct Plan {
   PlanKind kind = PlanKind::Resident;
@@ -69,6 +69,11 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p);
 int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, size_t n,
                size_t dim, size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out,
                float *d_out_pts, float init, hipStream_t st);
+// Windowed-update (NPDU) runs: plan, and enqueue on device data (start d_start[0]).
+int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p);
+int launch_npdu(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, size_t n,
+                size_t dim, size_t m, size_t window, const uint64_t *d_start, uint64_t *d_out,
+                float *d_out_pts, hipStream_t st);
 int status_code(uint32_t st);
 struct SpatialScratch *new_spatial_scratch();
 int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,

@@ -154,6 +154,35 @@ py::array_t<size_t> bucket_fps_kdline_sampling_py(CArray points, size_t n_sample
   return out;
 }
 
+// src/lib.cpp:342-366 (dispatch) and :272-340 (the loop, on the GPU).
+py::array_t<size_t> fps_npdu_sampling_py(CArray points, size_t n_samples, size_t k,
+                                         py::object start_idx_obj) {
+  StartIndex start_idx = to_start_index(start_idx_obj);
+  check_py_input(points, n_samples, start_idx);
+  if (start_idx.type == StartIndex::ARRAY) {
+    PyErr_SetString(PyExc_NotImplementedError, "Array of start indices not implemented yet");
+    throw py::error_already_set();
+  }
+  const py::ssize_t P = points.shape(0);
+  const py::ssize_t C = points.shape(1);
+  if (P <= 0 || C <= 0)
+    throw py::value_error("points must be a 2D array with at least one column");
+  if (start_idx.single_idx >= static_cast<size_t>(P))
+    throw py::value_error("start_idx out of range");
+  py::array_t<size_t> out(n_samples);
+  if (n_samples == 0) return out;
+  const float *data = points.data();
+  uint64_t *optr = reinterpret_cast<uint64_t *>(out.mutable_data());
+  int rc;
+  {
+    py::gil_scoped_release nogil;
+    rc = fpsx_fps_npdu_sampling(data, (size_t)P, (size_t)C, n_samples, k, start_idx.single_idx,
+                                optr);
+  }
+  if (rc != FPSX_OK) raise_rc(rc, "fps_npdu_sampling");
+  return out;
+}
+
 // Batched form (no counterpart in the reference): every cloud sampled as
 // fps_sampling would sample it alone, all in one persistent GPU launch.
 py::list fps_sampling_batch_py(py::list clouds, std::vector<size_t> n_samples,
@@ -218,6 +247,7 @@ PYBIND11_MODULE(_fpsample, m) {
            :toctree: _generate
 
            _fps_sampling
+           _fps_npdu_sampling
            _bucket_fps_kdline_sampling
     )pbdoc";
 
@@ -228,6 +258,18 @@ PYBIND11_MODULE(_fpsample, m) {
                 points (np.ndarray[float32, 2D]): N x C point array.
                 n_samples (int): number of samples to pick.
                 start_idx (int or np.ndarray[uint64, 1D]): initial index or indices.
+            Returns:
+                np.ndarray[uint64]: sampled point indices.
+    )pbdoc");
+
+  m.def("_fps_npdu_sampling", &fps_npdu_sampling_py, py::arg("points"), py::arg("n_samples"),
+        py::arg("k"), py::arg("start_idx"), R"pbdoc(
+            FPS with Nearest Point Distance Update on the GPU.
+            Args:
+                points (np.ndarray[float32, 2D]): N x C point array.
+                n_samples (int): number of samples to pick.
+                k (int): window of indices updated around each pick.
+                start_idx (int): initial index to start FPS.
             Returns:
                 np.ndarray[uint64]: sampled point indices.
     )pbdoc");

@@ -70,6 +70,10 @@ def lib() -> ctypes.CDLL:
         L.fpsx_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
         ip = ctypes.POINTER(ctypes.c_int)
         L.fpsx_plan_info.argtypes = [sz, sz, ctypes.c_int, sz, ip, ip, ip, ip, ip]
+        L.fpsx_fps_npdu_sampling_device.argtypes = [vp, sz, sz, sz, sz, sz, sz, vp, vp, vp, vp]
+        L.fpsx_fps_npdu_sampling_device.restype = ctypes.c_int
+        L.fpsx_npdu_plan_info.argtypes = [sz, sz, ip, ip, ip, ip]
+        L.fpsx_npdu_plan_info.restype = ctypes.c_int
         _fpsx = L
     return _fpsx
 
@@ -148,6 +152,26 @@ def fps_sampling_device_strided(points, stride_point: int, stride_dim: int, n_po
         n_starts, _addr(out), _addr(out_points) if out_points is not None else None, tie, init,
         stream)
     _raise(rc, "fpsx_fps_sampling_device_strided")
+
+
+def fps_npdu_sampling_device(points, stride_point: int, stride_dim: int, n_points: int,
+                             dim: int, n_samples: int, w: int, start, out, out_points=None,
+                             stream: Optional[int] = None) -> None:
+    """Enqueue windowed-update FPS (fps_npdu_sampling semantics) on device data;
+    `start` is a device array holding the one start index."""
+    rc = lib().fpsx_fps_npdu_sampling_device(
+        _addr(points), stride_point, stride_dim, n_points, dim, n_samples, w, _addr(start),
+        _addr(out), _addr(out_points) if out_points is not None else None, stream)
+    _raise(rc, "fpsx_fps_npdu_sampling_device")
+
+
+def npdu_plan_info(n_points: int, dim: int) -> dict:
+    """The windowed-update kernel chosen for a cloud (kernel_dim 0: any dimension)."""
+    v = [ctypes.c_int() for _ in range(4)]
+    _raise(lib().fpsx_npdu_plan_info(n_points, dim, *[ctypes.byref(x) for x in v]),
+           "fpsx_npdu_plan_info")
+    kdim, block, ppt, grid = (x.value for x in v)
+    return dict(kernel_dim=kdim, block=block, ppt=ppt, grid=grid)
 
 
 def _tensor_view(t):

@@ -137,6 +137,41 @@ int fpsx_fps_sampling_device_strided(const float *d_points, size_t stride_point,
                                      float *d_out_points, int tie, float init, void *stream);
 
 /*
+ * FPS with nearest-point-distance-updating (NPDU).
+ * Replaces fps_npdu_sampling(points, n_samples, k, start_idx)
+ * (src/lib.cpp:272-340): the start updates the min-distance of every point;
+ * every later pick p updates only indices p - w/2 .. p + w/2 (the window
+ * shifted to stay inside [0, n_points-1], src/lib.cpp:296-300); each pick is
+ * the largest min-distance over all points, lowest index on ties.
+ *   points    host array, n_points x dim float32, row-major (C order)
+ *   w         window (the reference's k; any w >= 2*n_points covers the cloud)
+ *   out       host array of n_samples indices
+ * Errors: FPSX_EBADDIM (dim == 0), FPSX_EBADSTART (start_idx >= n_points),
+ * FPSX_EINVAL (n_samples == 0 or > n_points, null pointer).
+ */
+int fpsx_fps_npdu_sampling(const float *points, size_t n_points, size_t dim, size_t n_samples,
+                           size_t w, size_t start_idx, uint64_t *out);
+
+/*
+ * Same computation on device data, enqueued on `stream` (NULL: the library's
+ * stream).  Coordinate j of point i at d_points[i*stride_point + j*stride_dim];
+ * d_start is a device array holding the one start index; d_out_points (may be
+ * NULL) receives the picks' coordinates, n_samples x dim row-major.
+ */
+int fpsx_fps_npdu_sampling_device(const float *d_points, size_t stride_point, size_t stride_dim,
+                                  size_t n_points, size_t dim, size_t n_samples, size_t w,
+                                  const uint64_t *d_start, uint64_t *d_out, float *d_out_points,
+                                  void *stream);
+
+/*
+ * Kernel the windowed-update entries would use for a cloud: kernel_dim 3
+ * (coordinates in registers) or 0 (any dimension), workgroup size, points per
+ * thread and grid.  FPSX_NPDU_PLAN="D:B:PPT" in the environment forces a shape.
+ */
+int fpsx_npdu_plan_info(size_t n_points, size_t dim, int *kernel_dim, int *block, int *ppt,
+                        int *grid);
+
+/*
  * Batched FPS: many independent clouds sampled in one persistent launch.
  * The clouds are concatenated in one SoA device array (coordinate j of point p
  * at d_points[j*ld + p]); cloud c holds points offsets[c] .. offsets[c+1]-1

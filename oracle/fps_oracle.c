@@ -25,6 +25,9 @@
  *   oracle_kdline_exhaustive equality with oracle_kdline_quickfps and the
  *                         golden vectors (the exactness argument of bucket
  *                         pruning, checked on tie-heavy lattices).
+ *   oracle_npdu           golden vectors of the reference _fps_npdu_sampling;
+ *                         with a window covering the cloud it must equal
+ *                         oracle_fps_ex(lowest) (the full update case).
  */
 #include <float.h>
 #include <math.h>
@@ -121,6 +124,63 @@ ORACLE_API int oracle_fps_ex(const float *pts, size_t n, size_t d, size_t m,
             if (picked_dist)
                 picked_dist[s] = bestv;
         }
+        out[s] = last;
+    }
+    free(dmin);
+    return 0;
+}
+
+/*
+ * FPS with nearest-point-distance-updating (NPDU), src/lib.cpp:272-340: the
+ * first pick updates every point; every later pick p updates only the window
+ * of `w + 1` indices centred on p (src/lib.cpp:296-300: start = p - w/2,
+ * end = p + w/2, shifted to stay inside [0, n-1]); the argmax is over all
+ * points with a strict `>` (lowest index among equal maxima, src/lib.cpp:310-315).
+ */
+ORACLE_API int oracle_npdu(const float *pts, size_t n, size_t d, size_t m, size_t w,
+                           size_t start, uint64_t *out)
+{
+    if (m == 0)
+        return 0;
+    float *dmin = (float *)malloc(n * sizeof(float));
+    if (!dmin)
+        return -1;
+    for (size_t i = 0; i < n; ++i)
+        dmin[i] = INFINITY;
+    for (size_t i = 0; i < n; ++i) {
+        float dist = sqdist_row(pts, d, i, start);
+        if (dist < dmin[i])
+            dmin[i] = dist;
+    }
+    out[0] = start;
+    size_t last = start;
+    for (size_t s = 1; s < m; ++s) {
+        int64_t hw = (int64_t)(w / 2);
+        int64_t lo = (int64_t)last - hw;
+        int64_t hi = (int64_t)last + hw;
+        if (lo < 0) {
+            hi -= lo;
+            lo = 0;
+        }
+        if (hi >= (int64_t)n) {
+            lo = lo - (hi - (int64_t)n + 1);
+            if (lo < 0)
+                lo = 0;
+            hi = (int64_t)n - 1;
+        }
+        for (int64_t i = lo; i <= hi; ++i) {
+            float dist = sqdist_row(pts, d, (size_t)i, last);
+            if (dist < dmin[i])
+                dmin[i] = dist;
+        }
+        size_t best = 0;
+        float bestv = -1.0f;
+        for (size_t i = 0; i < n; ++i)
+            if (dmin[i] > bestv) {
+                bestv = dmin[i];
+                best = i;
+            }
+        last = best;
         out[s] = last;
     }
     free(dmin);

@@ -59,6 +59,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_fps.restype = ctypes.c_int
         L.oracle_kdline_perm.argtypes = [_f32p, sz, sz, sz, _u64p, _u64p, _u64p]
         L.oracle_kdline_perm.restype = ctypes.c_long
+        L.oracle_npdu.argtypes = [_f32p, sz, sz, sz, sz, sz, _u64p]
+        L.oracle_npdu.restype = ctypes.c_int
         for name in ("oracle_kdline_quickfps", "oracle_kdline_exhaustive"):
             fn = getattr(L, name)
             fn.argtypes = [_f32p, sz, sz, sz, sz, sz, _u64p]
@@ -138,6 +140,17 @@ def kdline_exhaustive(pc, n_samples, h, start_idx=0):
     return _kdline(lib().oracle_kdline_exhaustive, pc, n_samples, h, start_idx)
 
 
+def npdu(pc: np.ndarray, n_samples: int, w: int, start_idx: int = 0) -> np.ndarray:
+    """FPS with windowed updates (src/lib.cpp:272-340)."""
+    pts = _rows(pc)
+    out = np.zeros(n_samples, dtype=np.uint64)
+    rc = lib().oracle_npdu(pts.ctypes.data_as(_f32p), pts.shape[0], pts.shape[1], n_samples,
+                           int(w), int(start_idx), out.ctypes.data_as(_u64p))
+    if rc != 0:
+        raise MemoryError("oracle_npdu failed")
+    return out
+
+
 def fps_numpy(pc: np.ndarray, n_samples: int, start_idx=0, lowest: bool = False,
               init: float = float("inf")) -> np.ndarray:
     """Vectorised numpy restatement of src/lib.cpp:111-246 (small inputs only)."""
@@ -183,6 +196,11 @@ class Reference:
         if isinstance(start_idx, (list, tuple, np.ndarray)):
             start_idx = np.array(start_idx, dtype=np.uint64)
         return self.module._fps_sampling(pc, n_samples, start_idx)
+
+    def fps_npdu(self, pc: np.ndarray, n_samples: int, w: int, start_idx: int) -> np.ndarray:
+        # what src/fpsample/__init__.py:98-103 hands to _fps_npdu_sampling
+        pc = np.ascontiguousarray(pc, dtype=np.float32)
+        return self.module._fps_npdu_sampling(pc, n_samples, w, start_idx)
 
     def bucket_fps_kdline(self, pc: np.ndarray, n_samples: int, h: int,
                           start_idx: int) -> np.ndarray:

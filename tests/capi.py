@@ -42,6 +42,14 @@ class CApi:
         L.fpsx_kdline_permutation.restype = ctypes.c_long
         L.fpsx_plan_info.argtypes = [_sz, _sz, ctypes.c_int, _ip, _ip, _ip, _ip, _ip]
         L.fpsx_plan_info.restype = ctypes.c_int
+        L.fpsx_fps_npdu_sampling.argtypes = [_f32p, _sz, _sz, _sz, _sz, _sz, _u64p]
+        L.fpsx_fps_npdu_sampling.restype = ctypes.c_int
+        L.fpsx_fps_npdu_sampling_device.argtypes = [ctypes.c_void_p, _sz, _sz, _sz, _sz, _sz,
+                                                    _sz, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.fpsx_fps_npdu_sampling_device.restype = ctypes.c_int
+        L.fpsx_npdu_plan_info.argtypes = [_sz, _sz, _ip, _ip, _ip, _ip]
+        L.fpsx_npdu_plan_info.restype = ctypes.c_int
         self.L = L
 
     def device_count(self) -> int:
@@ -62,6 +70,23 @@ class CApi:
                                       st.ctypes.data_as(_u64p), st.size,
                                       out.ctypes.data_as(_u64p))
         return rc, out[: int(m)]
+
+    def npdu(self, pc, m, w, start, dim=None):
+        """(rc, picks) of fpsx_fps_npdu_sampling on a row-major host array."""
+        pc = np.ascontiguousarray(pc, dtype=np.float32)
+        d = pc.shape[1] if dim is None else dim
+        out = np.zeros(max(int(m), 1), dtype=np.uint64)
+        rc = self.L.fpsx_fps_npdu_sampling(pc.ctypes.data_as(_f32p), pc.shape[0], d, m, w, start,
+                                           out.ctypes.data_as(_u64p))
+        return rc, out[: int(m)]
+
+    def npdu_plan_info(self, n, d):
+        v = [ctypes.c_int() for _ in range(4)]
+        rc = self.L.fpsx_npdu_plan_info(n, d, *[ctypes.byref(x) for x in v])
+        if rc != 0:
+            raise RuntimeError(self.last_error())
+        kdim, block, ppt, grid = (x.value for x in v)
+        return dict(kernel_dim=kdim, block=block, ppt=ppt, grid=grid)
 
     def kdline(self, pc, m, start, h, dim=None):
         pc = np.ascontiguousarray(pc, dtype=np.float32)

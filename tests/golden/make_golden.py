@@ -9,7 +9,7 @@ container where /root/reference exists:
 
 Each fixture holds: pc (float32 rows) -- or pc_from, the name of the fixture
 whose pc it shares --, m, start (uint64 array), kind ("fps" or "kdline"),
-h (kdline height, 0 for fps), out (uint64 picks).  tests/golden/__init__.py
+h (kdline height or npdu window, 0 for fps), out (uint64 picks).  tests/golden/__init__.py
 loads them.
 """
 from __future__ import annotations
@@ -67,6 +67,11 @@ def cases():
     for d in (1, 2, 5, 8):
         yield f"kd_dim{d}_h4", rng.random((900, d)).astype(np.float32), 300, [4], "kdline", 4
     yield "kd_same_h2", np.ones((40, 3), np.float32), 40, [3], "kdline", 2
+    # NPDU (h holds the window w)
+    for w, s in ((64, 0), (16, 2000), (4095, 17), (3, 4095)):
+        yield f"npdu_c1_w{w}_s{s}", c1(), 1024, [s], "npdu", w
+    yield "npdu_lattice_w40", lattice(3000, 6, 0), 800, [5], "npdu", 40
+    yield "npdu_sorted_w32", np.sort(rng.random((2000, 2)).astype(np.float32), axis=0), 500, [100], "npdu", 32
 
 
 def main():
@@ -80,6 +85,8 @@ def main():
         if kind == "fps":
             st = start[0] if len(start) == 1 else list(start)
             out = R.fps_sampling(pc, m, st)
+        elif kind == "npdu":
+            out = R.fps_npdu(pc, m, h, start[0])
         else:
             out = R.bucket_fps_kdline(pc, m, h, start[0])
         data = dict(m=np.int64(m), start=np.array(start, np.uint64), kind=np.array(kind),

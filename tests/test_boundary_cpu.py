@@ -177,3 +177,55 @@ def test_pybind_validation_matches_reference(reference):
             reference.module._fps_sampling(*args)
         assert type(ours.value) is type(theirs.value)
         assert str(ours.value) == str(theirs.value)
+
+
+def test_npdu_validation_matches_reference(reference):
+    """_fps_npdu_sampling: same exception type and message as the reference."""
+    from fpsample_amd import _fpsample as X
+
+    pc = np.random.rand(10, 3).astype(np.float32)
+    bad = [
+        (np.zeros(10, np.float32), 3, 4, 0),
+        (np.zeros((10, 0), np.float32), 3, 4, 0),
+        (pc, 11, 4, 0),
+        (pc, 3, 4, 10),
+        (pc, 3, 4, np.array([0, 1], np.uint64)),
+        (pc, 2, 4, np.array([0, 1, 2], np.uint64)),
+        (pc, 3, 4, 1.5),
+    ]
+    for args in bad:
+        with pytest.raises(Exception) as ours:
+            X._fps_npdu_sampling(*args)
+        with pytest.raises(Exception) as theirs:
+            reference.module._fps_npdu_sampling(*args)
+        assert type(ours.value) is type(theirs.value), args
+        assert str(ours.value) == str(theirs.value)
+    assert X._fps_npdu_sampling(pc, 0, 4, 0).shape == (0,)
+
+
+def test_npdu_wrapper_window_default_and_asserts(monkeypatch):
+    seen = {}
+
+    def fake(pc, n, w, start):
+        seen.update(pc=pc, n=n, w=w, start=start)
+        return np.zeros(n, np.uint64)
+
+    monkeypatch.setattr(F, "_fps_npdu_sampling", fake)
+    pc = np.random.rand(1000, 3)
+    np.random.seed(9)
+    F.fps_npdu_sampling(pc, 100)
+    np.random.seed(9)
+    assert seen["start"] == np.random.randint(0, 1000)
+    assert seen["w"] == int(1000 / 100 * 16)
+    assert seen["pc"].dtype == np.float32 and seen["pc"].flags.c_contiguous
+    with pytest.warns(UserWarning, match="k is too large, set to 999"):
+        F.fps_npdu_sampling(pc, 5, start_idx=0)
+    assert seen["w"] == 999
+    F.fps_npdu_sampling(pc, 5, w=3, start_idx=2)
+    assert (seen["w"], seen["start"]) == (3, 2)
+    with pytest.raises(AssertionError, match="n_pts should be >= n_samples"):
+        F.fps_npdu_sampling(pc, 1001)
+    with pytest.raises(AssertionError, match="start_idx should be None or 0 <= start_idx < n_pts"):
+        F.fps_npdu_sampling(pc, 3, start_idx=1000)
+    with pytest.raises(TypeError):
+        F.fps_npdu_sampling(pc, 3, start_idx=[0, 1])   # `0 <= list` in the reference's assert

@@ -1,0 +1,150 @@
+"""GPU parity of windowed-update FPS (fps_npdu_sampling, src/lib.cpp:272-340).
+
+Bit-exact against the reference's golden vectors and against the oracle
+(oracle.npdu, pinned by those vectors and by hand-derived known answers).
+FPSX_NPDU_PLAN forces every kernel shape: the 3-D kernel with coordinates in
+registers and the any-dimension kernel that reads only the window, each with
+one and with several workgroups.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture
+def npdu_plan(monkeypatch):
+    def set_plan(p):
+        if p is None:
+            monkeypatch.delenv("FPSX_NPDU_PLAN", raising=False)
+        else:
+            monkeypatch.setenv("FPSX_NPDU_PLAN", p)
+    yield set_plan
+    monkeypatch.delenv("FPSX_NPDU_PLAN", raising=False)
+
+
+def run(gpu, pc, m, w, s):
+    rc, out = gpu.npdu(pc, m, w, s)
+    assert rc == 0, f"rc={rc}: {gpu.last_error()}"
+    return out
+
+
+def test_golden_npdu_capi(gpu, golden):
+    n = 0
+    for name, g in golden.items():
+        if g["kind"] != "npdu":
+            continue
+        log(f"golden {name}")
+        np.testing.assert_array_equal(run(gpu, g["pc"], g["m"], g["h"], int(g["start"][0])),
+                                      g["out"], err_msg=name)
+        n += 1
+    assert n >= 6
+
+
+SHAPES = ["3:256:1", "3:256:4", "3:1024:1", "3:1024:4", "3:1024:16",
+          "0:256:1", "0:256:4", "0:1024:4", "0:1024:16", "0:1024:32"]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_npdu_shapes(gpu, npdu_plan, shape):
+    npdu_plan(shape)
+    rng = np.random.default_rng(41)
+    clouds = [("uniform", rng.random((5000, 3)).astype(np.float32)),
+              ("sorted_gauss", np.sort(rng.standard_normal((4000, 3)).astype(np.float32), axis=0)),
+              ("lattice_dups", np.repeat(rng.integers(0, 5, (1200, 3)), 3, axis=0).astype(np.float32))]
+    for name, pc in clouds:
+        n = len(pc)
+        log(f"npdu {shape} {name} plan={gpu.npdu_plan_info(n, 3)}")
+        for w, s in ((0, 0), (1, n - 1), (7, 123), (64, n // 2), (1000, 5), (10 ** 12, 9)):
+            np.testing.assert_array_equal(run(gpu, pc, 600, w, s), O.npdu(pc, 600, w, s),
+                                          err_msg=f"{shape} {name} w={w} s={s}")
+
+
+@pytest.mark.parametrize("d", [1, 2, 4, 5, 8, 13])
+def test_npdu_dimensions(gpu, d):
+    rng = np.random.default_rng(42 + d)
+    pc = rng.standard_normal((3000, d)).astype(np.float32)
+    log(f"npdu d={d} plan={gpu.npdu_plan_info(3000, d)}")
+    for w in (3, 48, 5000):
+        np.testing.assert_array_equal(run(gpu, pc, 700, w, 11), O.npdu(pc, 700, w, 11),
+                                      err_msg=f"d={d} w={w}")
+
+
+def test_npdu_edge_sizes(gpu):
+    rng = np.random.default_rng(43)
+    for n in (1, 2, 3, 63, 64, 65, 256, 257, 1024, 1025, 16384, 16385, 20000):
+        pc = rng.random((n, 3)).astype(np.float32)
+        for m in sorted({k for k in (1, 2, min(n, 5), min(n, 300)) if k <= n}):
+            log(f"npdu edge n={n} m={m}")
+            w = max(1, int(n / m * 16))
+            np.testing.assert_array_equal(run(gpu, pc, m, w, n - 1), O.npdu(pc, m, w, n - 1),
+                                          err_msg=f"n={n} m={m}")
+
+
+def test_npdu_special_values(gpu, npdu_plan):
+    rng = np.random.default_rng(44)
+    pc = (rng.random((3000, 3)) * 4e19).astype(np.float32)   # squares overflow to +inf
+    pc[[5, 700]] = np.nan
+    pc[9] = np.inf
+    for plan in (None, "0:1024:4"):
+        npdu_plan(plan)
+        np.testing.assert_array_equal(run(gpu, pc, 400, 32, 0), O.npdu(pc, 400, 32, 0))
+    same = np.ones((2000, 3), np.float32)                      # every distance 0: index order
+    np.testing.assert_array_equal(run(gpu, same, 50, 16, 3), O.npdu(same, 50, 16, 3))
+
+
+def test_npdu_large_multi_workgroup(gpu, npdu_plan):
+    # Several workgroups exchange candidates every pass; the window is the
+    # Python wrapper's default, n / m * 16.
+    rng = np.random.default_rng(45)
+    n, m = 200_000, 2048
+    pc = np.sort(rng.random((n, 3)).astype(np.float32), axis=0)
+    w = int(n / m * 16)
+    want = O.npdu(pc, m, w, 1234)
+    for plan in (None, "0:1024:32", "3:1024:4"):
+        npdu_plan(plan)
+        log(f"npdu large plan={plan} -> {gpu.npdu_plan_info(n, 3)}")
+        np.testing.assert_array_equal(run(gpu, pc, m, w, 1234), want, err_msg=str(plan))
+
+
+def test_npdu_python_api_and_device_entry(gpu):
+    import fpsample_amd as F
+    from fpsample_amd import device as D
+
+    rng = np.random.default_rng(46)
+    pc = rng.random((4096, 3)).astype(np.float64)
+    np.random.seed(3)
+    got = F.fps_npdu_sampling(pc, 1024)
+    np.random.seed(3)
+    s = np.random.randint(low=0, high=4096)
+    want = O.npdu(pc.astype(np.float32), 1024, int(4096 / 1024 * 16), s)
+    np.testing.assert_array_equal(got, want)
+    assert got.dtype == np.uint64
+    for d in (3, 6):
+        q = rng.standard_normal((5000, d)).astype(np.float32)
+        rows = D.DeviceArray.from_numpy(q)
+        ds = D.DeviceArray.from_numpy(np.array([77], np.uint64))
+        out = D.DeviceArray((500,), np.uint64)
+        opts = D.DeviceArray((500, d), np.float32)
+        D.fps_npdu_sampling_device(rows, d, 1, 5000, d, 500, 40, ds, out, opts)
+        D.synchronize()
+        D.device_status()
+        want = O.npdu(q, 500, 40, 77)
+        np.testing.assert_array_equal(out.numpy(), want, err_msg=f"d={d}")
+        np.testing.assert_array_equal(opts.numpy(), q[want.astype(np.int64)])
+
+
+def test_npdu_error_codes(gpu):
+    pc = np.zeros((10, 3), np.float32)
+    assert gpu.npdu(pc, 3, 4, 10)[0] == 2      # start out of range
+    assert gpu.npdu(pc, 11, 4, 0)[0] == 3      # n_samples > n_points
+    assert gpu.npdu(pc, 3, 4, 0, dim=0)[0] == 1

@@ -50,6 +50,8 @@ def test_golden_through_python_api(gpu, golden):
         if g["kind"] == "fps":
             arg = int(st[0]) if st.size == 1 else [int(x) for x in st]
             out = F.fps_sampling(g["pc"], g["m"], start_idx=arg)
+        elif g["kind"] == "npdu":
+            out = F.fps_npdu_sampling(g["pc"], g["m"], w=g["h"], start_idx=int(st[0]))
         else:
             out = F.bucket_fps_kdline_sampling(g["pc"], g["m"], g["h"], start_idx=int(st[0]))
         assert out.dtype == np.uint64

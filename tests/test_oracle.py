@@ -181,3 +181,69 @@ def test_rng_draw_is_numpy_legacy():
     # numpy versions, so the value after seed(42) is fixed.
     np.random.seed(42)
     assert np.random.randint(low=0, high=4096) == 3174
+
+
+# --- windowed-update FPS (NPDU), src/lib.cpp:272-340 -------------------------
+
+def test_golden_npdu(golden):
+    n = 0
+    for name, g in golden.items():
+        if g["kind"] != "npdu":
+            continue
+        out = O.npdu(g["pc"], g["m"], g["h"], int(g["start"][0]))
+        np.testing.assert_array_equal(out, g["out"], err_msg=name)
+        n += 1
+    assert n >= 6
+
+
+# Known answers derived by hand from src/lib.cpp:294-316 (1-D clouds).
+#  * w = 0: each pick updates only itself, so pick 2 is index 1 (d=100 kept),
+#    where full-update FPS would pick index 2.
+#  * w = 2 at the right edge: the window of pick 3 is shifted to [1, 3]
+#    (src/lib.cpp:300), which lowers index 1 to 1 and leaves index 2 (d=9) the
+#    farthest; a plain clamp to [2, 3] would pick index 1.
+#  * the mirror image at the left edge ([0, 2] rather than [0, 1]).
+NPDU_KATS = [
+    (np.array([[0], [10], [2], [11]], np.float32), 4, 0, 0, [0, 3, 1, 2]),
+    (np.array([[0], [19], [3], [20]], np.float32), 3, 2, 0, [0, 3, 2]),
+    (np.array([[20], [3], [19], [0]], np.float32), 3, 2, 3, [3, 0, 1]),
+]
+
+
+@pytest.mark.parametrize("pc,m,w,s,want", NPDU_KATS)
+def test_npdu_known_answers(pc, m, w, s, want):
+    assert O.npdu(pc, m, w, s).tolist() == want
+
+
+def test_npdu_known_answers_against_reference(reference):
+    for pc, m, w, s, want in NPDU_KATS:
+        assert reference.fps_npdu(pc, m, w, s).tolist() == want
+
+
+def test_npdu_planted_mistakes_are_caught():
+    # Full updates (vanilla FPS with lowest-index ties) and the highest-index
+    # tie rule are the plausible mistakes; the known answers reject both.
+    pc, m, w, s, want = NPDU_KATS[0]
+    assert O.fps_ex(pc, m, s, lowest=True).tolist() != want
+    pc = np.array([[0], [1], [1]], np.float32)
+    assert O.npdu(pc, 2, 0, 0).tolist() == [0, 1]
+    assert O.fps_ex(pc, 2, 0).tolist() == [0, 2]
+
+
+def test_npdu_full_window_is_lowest_tie_fps():
+    # A window covering the cloud makes every update global: vanilla FPS with
+    # the argmax's strict `>` (lowest index among equal maxima).
+    rng = np.random.default_rng(5)
+    for pc in (rng.random((700, 3)).astype(np.float32),
+               rng.integers(0, 4, (500, 2)).astype(np.float32)):
+        n = len(pc)
+        for w in (2 * n, 2 * n + 1, 10 ** 9):
+            np.testing.assert_array_equal(O.npdu(pc, 200, w, 7), O.fps_ex(pc, 200, 7, lowest=True))
+
+
+def test_npdu_against_reference_random(reference):
+    rng = np.random.default_rng(6)
+    for n, d, w in ((300, 3, 0), (1000, 3, 17), (800, 5, 64), (257, 1, 256), (600, 2, 5000)):
+        pc = rng.standard_normal((n, d)).astype(np.float32)
+        s = int(rng.integers(0, n))
+        np.testing.assert_array_equal(O.npdu(pc, n // 2, w, s), reference.fps_npdu(pc, n // 2, w, s))
