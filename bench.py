@@ -3,10 +3,13 @@
 
 A step is one vanilla FPS run (fps_sampling semantics, src/lib.cpp:188-246)
 over one synthetic cloud already resident in HBM: one call of
-fpsx_fps_sampling_device, i.e. the exchange-area clear and the single
-persistent FPS launch that makes all M-1 passes.  The kernel shape comes
-from the library's planner (per-pass time model); --autotune on times every
-shape first (untimed) instead.
+fpsx_fps_sampling_device, i.e. the exchange-area clear, for large 3-D clouds
+the spatial ordering of the points (bounding box, Morton codes, two radix
+sorts, gather), and the single persistent FPS launch that makes all M-1
+passes.  The kernel shape comes from the library's planner; --autotune on
+times every shape first (untimed) instead.  The npdu* workloads time
+fps_npdu_sampling (src/lib.cpp:272-340) the same way, through
+fpsx_fps_npdu_sampling_device on row-major device data.
 
 Default workload (the metric's configuration): one cloud of N = 1,048,576
 points, D = 3 float32, a mixture of 64 isotropic Gaussian clusters (sigma
@@ -47,6 +50,12 @@ def gen_uniform_4096(seed: int = 42) -> np.ndarray:
     """N=4096 uniform in [0,1)^3 from np.random.seed(42) (bench/test_bench.py:19-21)."""
     np.random.seed(seed)
     return np.random.rand(4096, 3).astype(np.float32)
+
+
+def gen_uniform_n(n: int, seed: int = 42) -> np.ndarray:
+    """N uniform in [0,1)^3 from np.random.seed(42) (bench/test_bench.py:19-21)."""
+    np.random.seed(seed)
+    return np.random.rand(n, 3).astype(np.float32)
 
 
 def gen_gauss_mixture(n: int = 1 << 20, seed: int = 1) -> np.ndarray:
@@ -106,20 +115,35 @@ WORKLOADS = {
     "lattice16m": dict(gen=gen_lattice_dups, m=4096,
                        desc="one cloud N=16777216 D=3 f32 lattice {0..255}^3, 25% duplicates, "
                             "M=4096, start_idx=0"),
+    # windowed-update FPS on the reference's own bench cases (bench/test_bench.py:43-46,
+    # 151-154): w is the Python wrapper's default int(N / M * 16)
+    "npdu4k": dict(gen=gen_uniform_4096, m=1024, kind="npdu",
+                   desc="fps_npdu_sampling, one cloud N=4096 D=3 f32 uniform [0,1)^3 seed 42, "
+                        "M=1024, w=64, start_idx=0"),
+    "npdu100k": dict(gen=lambda: gen_uniform_n(100_000), m=50_000, kind="npdu",
+                     desc="fps_npdu_sampling, one cloud N=100000 D=3 f32 uniform [0,1)^3 seed 42, "
+                          "M=50000, w=32, start_idx=0"),
 }
 
 
 # ---------------------------------------------------------------- CPU baseline
-def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float) -> dict:
+def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float, w: int = -1) -> dict:
+    """w < 0: fps_sampling; else fps_npdu_sampling with window w."""
     from oracle import oracle as O
 
     ref = O.load_reference()
     if ref is not None:
         kind = "reference"
-        run = lambda m: ref.fps_sampling(pc, m, 0)  # noqa: E731
+        if w < 0:
+            run = lambda m: ref.fps_sampling(pc, m, 0)  # noqa: E731
+        else:
+            run = lambda m: ref.fps_npdu(pc, m, w, 0)  # noqa: E731
     else:
         kind = "port"
-        run = lambda m: O.fps(pc, m, 0)  # noqa: E731
+        if w < 0:
+            run = lambda m: O.fps(pc, m, 0)  # noqa: E731
+        else:
+            run = lambda m: O.npdu(pc, m, w, 0)  # noqa: E731
     n = pc.shape[0]
     # calibrate on a few passes, then size the sample to about target_s seconds
     m_cal = max(2, min(m_full, int(2e8 // max(n, 1)) + 2))
@@ -338,16 +362,25 @@ def main() -> int:
     m = wl["m"]
     pc = wl["gen"]()
     n = pc.shape[0]
-    soa = np.ascontiguousarray(pc.T)  # (3, N): coordinate j of point i at j*N + i
-    d_pts = D.DeviceArray.from_numpy(soa)
+    npdu = wl.get("kind") == "npdu"
+    w = int(n / m * 16) if npdu else -1
     d_start = D.DeviceArray.from_numpy(np.zeros(1, np.uint64))
     d_out = D.DeviceArray((m,), np.uint64)
-    if args.autotune == "on":
-        D.autotune(d_pts, 1, n, n, 3)  # untimed: picks the kernel shape for this cloud
-    plan = D.plan_info(n, 3, n_samples=m)
+    if npdu:
+        d_pts = D.DeviceArray.from_numpy(np.ascontiguousarray(pc))  # row-major, as the reference
+        plan = dict(kind="npdu", **D.npdu_plan_info(n, 3))
 
-    def step():
-        D.fps_sampling_device(d_pts, n, 3, m, d_start, 1, d_out)
+        def step():
+            D.fps_npdu_sampling_device(d_pts, 3, 1, n, 3, m, w, d_start, d_out)
+    else:
+        soa = np.ascontiguousarray(pc.T)  # (3, N): coordinate j of point i at j*N + i
+        d_pts = D.DeviceArray.from_numpy(soa)
+        if args.autotune == "on":
+            D.autotune(d_pts, 1, n, n, 3)  # untimed: picks the kernel shape for this cloud
+        plan = D.plan_info(n, 3, n_samples=m)
+
+        def step():
+            D.fps_sampling_device(d_pts, n, 3, m, d_start, 1, d_out)
 
     for _ in range(args.warmup):
         step()
@@ -366,13 +399,18 @@ def main() -> int:
     D.device_status()
 
     out = d_out.numpy()
-    assert out[0] == 0 and len(np.unique(out)) == m, "picks failed the sanity check"
+    assert out[0] == 0 and (npdu or len(np.unique(out)) == m), "picks failed the sanity check"
 
     wall = t1 - t0
     ms_per_step = wall / args.steps * 1e3
     value = m * args.steps / wall
     kernel_s = kernel_ms / max(launches, 1) / 1e3
-    alg_bytes = BYTES_PER_POINT_PASS * n * (m - 1)
+    if npdu:
+        # per pass: the window's w+1 points updated (20 B each) and the argmax
+        # over every min-distance (4 B per point), src/lib.cpp:302-315
+        alg_bytes = (20 * (min(w, n - 1) + 1) + 4 * n) * (m - 1)
+    else:
+        alg_bytes = BYTES_PER_POINT_PASS * n * (m - 1)
     achieved = alg_bytes / kernel_s / 1e9
     result = {
         "metric": "samples/s at N=1M,M=16K (+% roofline)" if args.workload == "gauss1m"
@@ -403,7 +441,7 @@ def main() -> int:
         "cpu_baseline": None,
     }
     if args.cpu_baseline == "on":
-        result["cpu_baseline"] = cpu_baseline(pc, m, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(pc, m, args.cpu_seconds, w)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -36,7 +36,8 @@ __global__ void __launch_bounds__(B) fps_npdu(NpduParams a) {
   const int wave = tid >> 6;
   const uint32_t G = gridDim.x;
   const uint32_t g = blockIdx.x;
-  const uint32_t wbase = g * (uint32_t)(B * PPT) + (uint32_t)(wave * 64 * PPT);
+  const uint32_t wbase =
+      __builtin_amdgcn_readfirstlane(g * (uint32_t)(B * PPT) + (uint32_t)(wave * 64 * PPT));
   const uint32_t wlast = wbase + (uint32_t)(64 * PPT - 1);
   const uint32_t n = f.n;
 
@@ -108,9 +109,13 @@ __global__ void __launch_bounds__(B) fps_npdu(NpduParams a) {
       // registers across the whole pass loop.
       uint32_t ib = wbase + (uint32_t)lane;
       asm volatile("" : "+v"(ib));
+      // Slots the window reaches (uniform): k_lo .. k_hi.
+      const int k_lo = (int)((lo > wbase ? lo - wbase : 0u) >> 6);
+      const int k_hi = (int)((hi - wbase) >> 6);
       if constexpr (D > 0) {
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
+          if (k < k_lo || k > k_hi) continue;
           const uint32_t i = ib + (uint32_t)(k * 64);
           const float d = sqdist<D>(c[k], pv);
           dm[k] = (i >= lo && i <= hi) ? min_keep(d, dm[k]) : dm[k];
@@ -122,11 +127,12 @@ __global__ void __launch_bounds__(B) fps_npdu(NpduParams a) {
         const float *pp = f.pts + (size_t)pidx * f.sp;
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
+          if (k < k_lo || k > k_hi) continue;
           const uint32_t i = ib + (uint32_t)(k * 64);
-          const bool in = i >= lo && i <= hi;
-          if (__ballot(in) != 0 && in) {
+          if (i >= lo && i <= hi) {
             const float *q = f.pts + (size_t)i * f.sp;
             float acc = 0.0f;
+#pragma unroll 4
             for (uint32_t j = 0; j < f.dim; ++j) {
               const float dj = q[(size_t)j * f.ld] - pp[(size_t)j * f.ld];
               acc = (j == 0) ? dj * dj : acc + dj * dj;
@@ -196,7 +202,7 @@ __global__ void __launch_bounds__(B) fps_npdu(NpduParams a) {
 #define FPSX_NPDU(D, B, P) {D, B, P, kTieLowest, (const void *)&fps_npdu<D, B, P>}
 
 static const KernelEntry kNpdu[] = {
-    FPSX_NPDU(3, 256, 1),   FPSX_NPDU(3, 256, 2),   FPSX_NPDU(3, 256, 4),
+    FPSX_NPDU(3, 256, 1),   FPSX_NPDU(3, 256, 2),   FPSX_NPDU(3, 256, 4),   FPSX_NPDU(3, 256, 16),
     FPSX_NPDU(3, 1024, 1),  FPSX_NPDU(3, 1024, 2),  FPSX_NPDU(3, 1024, 4),
     FPSX_NPDU(3, 1024, 8),  FPSX_NPDU(3, 1024, 16), FPSX_NPDU(0, 256, 1),
     FPSX_NPDU(0, 256, 4),   FPSX_NPDU(0, 1024, 1),  FPSX_NPDU(0, 1024, 4),

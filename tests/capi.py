@@ -40,7 +40,7 @@ class CApi:
         L.bucket_fps_kdline.restype = ctypes.c_int
         L.fpsx_kdline_permutation.argtypes = [_f32p, _sz, _sz, _sz, _u64p, _u64p, _u64p]
         L.fpsx_kdline_permutation.restype = ctypes.c_long
-        L.fpsx_plan_info.argtypes = [_sz, _sz, ctypes.c_int, _ip, _ip, _ip, _ip, _ip]
+        L.fpsx_plan_info.argtypes = [_sz, _sz, ctypes.c_int, _sz, _ip, _ip, _ip, _ip, _ip]
         L.fpsx_plan_info.restype = ctypes.c_int
         L.fpsx_fps_npdu_sampling.argtypes = [_f32p, _sz, _sz, _sz, _sz, _sz, _u64p]
         L.fpsx_fps_npdu_sampling.restype = ctypes.c_int

@@ -50,7 +50,7 @@ def test_golden_npdu_capi(gpu, golden):
     assert n >= 6
 
 
-SHAPES = ["3:256:1", "3:256:4", "3:1024:1", "3:1024:4", "3:1024:16",
+SHAPES = ["3:256:1", "3:256:4", "3:256:16", "3:1024:1", "3:1024:4", "3:1024:8", "3:1024:16",
           "0:256:1", "0:256:4", "0:1024:4", "0:1024:16", "0:1024:32"]
 
 
