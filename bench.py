@@ -170,7 +170,7 @@ def _read_traffic(workload: str, plan: dict):
     """HBM bytes per launch of the FPS kernel from the committed rocprofv3
     counter summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known
     size), when that summary was taken with the plan this run uses."""
-    path = os.path.join(ROOT, "profiles", f"fps_kernel_{workload}_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"fps_{plan.get('kind')}_{workload}_traffic.txt")
     if not os.path.exists(path):
         return None
     kv = {}

@@ -64,13 +64,37 @@ __global__ void __launch_bounds__(kBoxThreads)
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void bbox_final(const float *__restrict__ part, uint32_t nparts, float *__restrict__ box) {
-  if (threadIdx.x >= 6) return;
-  const int j = threadIdx.x;
-  float v = (j < 3) ? INFINITY : -INFINITY;
-  for (uint32_t b = 0; b < nparts; ++b)
-    v = (j < 3) ? fminf(v, part[b * 6 + j]) : fmaxf(v, part[b * 6 + j]);
-  box[j] = v;
+// One block of kBoxThreads: thread t folds partials t, t + kBoxThreads, ...,
+// then a tree reduction in LDS (the partials are few, the latency matters).
+__global__ void __launch_bounds__(kBoxThreads)
+    bbox_final(const float *__restrict__ part, uint32_t nparts, float *__restrict__ box) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY};
+  float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t b = threadIdx.x; b < nparts; b += kBoxThreads) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      lo[j] = fminf(lo[j], part[b * 6 + j]);
+      hi[j] = fmaxf(hi[j], part[b * 6 + 3 + j]);
+    }
+  }
+  __shared__ float s[6][kBoxThreads];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    s[j][threadIdx.x] = lo[j];
+    s[3 + j][threadIdx.x] = hi[j];
+  }
+  __syncthreads();
+  for (int w = kBoxThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        s[j][threadIdx.x] = fminf(s[j][threadIdx.x], s[j][threadIdx.x + w]);
+        s[3 + j][threadIdx.x] = fmaxf(s[3 + j][threadIdx.x], s[3 + j][threadIdx.x + w]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) box[threadIdx.x] = s[threadIdx.x][0];
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -154,7 +178,7 @@ int spatial_order(const float *pts, uint64_t sp, uint64_t ld, uint32_t n, uint32
   const unsigned nb = (unsigned)std::min<uint64_t>(kBoxBlocks, (n + kBoxThreads - 1) / kBoxThreads);
   hipLaunchKernelGGL(bbox_partial, dim3(nb), dim3(kBoxThreads), 0, st, pts, sp, ld, n,
                      (float *)S.part.p);
-  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, st, (const float *)S.part.p, nb,
+  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBoxThreads), 0, st, (const float *)S.part.p, nb,
                      (float *)S.box.p);
   const unsigned g = (n + 255) / 256;
   hipLaunchKernelGGL(morton_codes, dim3(g), dim3(256), 0, st, pts, sp, ld, n,
