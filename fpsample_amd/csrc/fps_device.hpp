@@ -231,6 +231,15 @@ __device__ __forceinline__ bool exchange(uint64_t *base, uint32_t R, uint32_t pi
     if (lane == NW) xw[NW][wave] = wave_ok ? 1 : 0;
   }
   lds_barrier();
+  if (P == 1) {
+    // One polling wave: its winner is the result; every word is read at once.
+    int x[NW + 1];
+#pragma unroll
+    for (int k = 0; k <= NW; ++k) x[k] = xw[k][0];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) win[k] = x[k];
+    return x[NW] != 0;
+  }
   bool ok_all = true;
   for (int r = 0; r < P; ++r) ok_all = ok_all && (xw[NW][r] != 0);
   const int v3 = (lane < P) ? xw[0][lane] : INT_MIN;
@@ -242,18 +251,20 @@ __device__ __forceinline__ bool exchange(uint64_t *base, uint32_t R, uint32_t pi
   return ok_all;
 }
 
-// Records pick s: lane 0 of the calling wave writes the index, lanes j < D the
-// coordinates when the caller asked for them (the fused `pc[idx]` gather).
+// Records pick s: lane 0 of the calling wave writes the index and, when the
+// caller asked for them, the coordinates (the fused `pc[idx]` gather).  The
+// coordinates are stored one by one from lane 0: selecting pv[j] by lane
+// number would index pv dynamically, and the compiler would then move pv
+// out of registers into LDS for the whole kernel.
 template <int D>
 __device__ __forceinline__ void record_pick(const FpsParams &a, uint32_t s, uint32_t pidx,
                                             const float (&pv)[D]) {
-  const int lane = threadIdx.x & 63;
-  if (lane == 0) a.out[s] = pidx;
-  if (a.out_pts != nullptr && lane < D) {
-    float v = pv[0];
+  if ((threadIdx.x & 63) == 0) {
+    a.out[s] = pidx;
+    if (a.out_pts != nullptr) {
 #pragma unroll
-    for (int j = 1; j < D; ++j) v = (lane == j) ? pv[j] : v;
-    a.out_pts[(size_t)s * D + lane] = v;
+      for (int j = 0; j < D; ++j) a.out_pts[(size_t)s * D + j] = pv[j];
+    }
   }
 }
 
