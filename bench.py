@@ -25,7 +25,11 @@ the 8 TB/s HBM peak.  cpu_baseline: the reference's fps_sampling compiled
 from its own sources (oracle/_ref) -- or the oracle's C restatement where
 that build is absent -- timed single-threaded on this host over a bounded
 number of samples of the same cloud (its per-sample cost does not depend on
-how many samples are drawn).
+how many samples are drawn).  The same CPU leg reports accuracy_vs_fp64
+(index agreement and coverage-radius relative error of the GPU picks against
+the greedy loop in double precision) and, for the 1 M-point cloud,
+bucket_kdline: the bucket entry (h = 9) on the GPU beside the reference's
+bucket sampler on one host core, with their picks compared.
 """
 from __future__ import annotations
 
@@ -164,6 +168,46 @@ def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float, w: int = -1) -> d
                   f"{dt:.2f} s single-threaded on {platform.processor() or platform.machine()} "
                   f"(os.cpu_count()={os.cpu_count()})",
     }
+
+
+def accuracy_and_bucket(pc: np.ndarray, gpu_picks: np.ndarray, m_full: int, prefix: int = 1024,
+                        bucket_h: int = 9) -> dict:
+    """CPU-leg checks of the GPU result (the oracle only as the checker):
+    index agreement and coverage-radius relative error of the first `prefix`
+    GPU picks against the same greedy loop in double precision; and the
+    bucket entry (bucket_fps_kdline, h=9) timed on the GPU beside the
+    reference's own bucket sampler on one host core, with their picks
+    compared."""
+    from oracle import oracle as O
+
+    import fpsample_amd as F
+
+    k = min(prefix, gpu_picks.size)
+    acc = O.accuracy_vs_f64(pc, gpu_picks[:k], int(gpu_picks[0]))
+    acc["against"] = "double-precision greedy loop (CPU), same start"
+    out = {"accuracy_vs_fp64": acc}
+    ref = O.load_reference()
+    F.bucket_fps_kdline_sampling(pc, m_full, bucket_h, start_idx=0)  # warm-up (threads, buffers)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        gpu = F.bucket_fps_kdline_sampling(pc, m_full, bucket_h, start_idx=0)
+        ts.append(time.perf_counter() - t0)
+    b = {"h": bucket_h, "n_samples": m_full,
+         "gpu": {"value": round(m_full / min(ts), 3), "unit": "samples/s",
+                 "ms": round(min(ts) * 1e3, 3),
+                 "note": "host arrays in and out: KD-line permutation on host threads, "
+                         "upload, GPU sampling, download"}}
+    if ref is not None:
+        t0 = time.perf_counter()
+        want = ref.bucket_fps_kdline(pc, m_full, bucket_h, 0)
+        dt = time.perf_counter() - t0
+        b["cpu_reference"] = {"value": round(m_full / dt, 3), "unit": "samples/s",
+                              "ms": round(dt * 1e3, 3), "cores": 1}
+        b["identical_picks"] = bool(np.array_equal(gpu, want))
+        b["gpu_over_cpu"] = round(dt / min(ts), 2)
+    out["bucket_kdline"] = b
+    return out
 
 
 def _read_traffic(workload: str, plan: dict):
@@ -442,6 +486,12 @@ def main() -> int:
     }
     if args.cpu_baseline == "on":
         result["cpu_baseline"] = cpu_baseline(pc, m, args.cpu_seconds, w)
+        if args.workload == "gauss1m":
+            result.update(accuracy_and_bucket(pc, out, m))
+        elif not npdu and n <= 65536:
+            from oracle import oracle as O
+
+            result["accuracy_vs_fp64"] = O.accuracy_vs_f64(pc, out, 0)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -64,7 +64,7 @@ def build(verbose: bool = False, jobs: int = 4) -> None:
         cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", _obj(s)])
     for s in SOURCES_HOST:
         cmds.append([CXX, "-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-ffp-contract=off",
-                     "-Wall", "-I", CSRC, "-I", INCLUDE, "-c", os.path.join(CSRC, s),
+                     "-pthread", "-Wall", "-I", CSRC, "-I", INCLUDE, "-c", os.path.join(CSRC, s),
                      "-o", _obj(s)])
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for f in [ex.submit(_run, c, verbose) for c in cmds]:
@@ -72,7 +72,7 @@ def build(verbose: bool = False, jobs: int = 4) -> None:
     lib = os.path.join(HERE, "libfpsx.so")
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
-          *[_obj(s) for s in SOURCES_HIP + SOURCES_HOST], "-Wl,--no-undefined"], verbose)
+          *[_obj(s) for s in SOURCES_HIP + SOURCES_HOST], "-pthread", "-Wl,--no-undefined"], verbose)
     os.replace(tmp, lib)
 
     import pybind11

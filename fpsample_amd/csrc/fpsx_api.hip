@@ -826,6 +826,7 @@ FPSX_EXPORT long fpsx_kdline_permutation(const float *raw_data, size_t n_points,
   t_err.clear();
   if (dim == 0) return -(long)fail(FPSX_EBADDIM, "dim must be >= 1");
   if (!raw_data || !perm || n_points == 0) return -(long)fail(FPSX_EINVAL, "bad arguments");
+  if (n_points >= (size_t(1) << 31)) return -(long)fail(FPSX_ETOOBIG, "n_points must be < 2^31");
   std::vector<uint64_t> pv, lo, hi;
   if (kdline_permutation(raw_data, n_points, dim, height, pv, nullptr, &lo, &hi) != 0)
     return -(long)fail(FPSX_EINVAL, "kdline permutation failed");
@@ -848,12 +849,12 @@ FPSX_EXPORT int bucket_fps_kdline(const float *raw_data, size_t n_points, size_t
   if (n_samples == 0) return fail(FPSX_EINVAL, "n_samples must be >= 1");
   if (n_points >= (size_t(1) << 31)) return fail(FPSX_ETOOBIG, "n_points must be < 2^31");
   std::vector<uint64_t> perm;
-  std::vector<float> rows;
-  if (kdline_permutation(raw_data, n_points, dim, height, perm, &rows, nullptr, nullptr) != 0)
+  std::vector<float> cols;
+  if (kdline_permutation(raw_data, n_points, dim, height, perm, &cols, nullptr, nullptr) != 0)
     return fail(FPSX_EINVAL, "kdline permutation failed");
   std::vector<uint64_t> pos(n_samples);
   uint64_t s0 = start_idx;
-  int rc = fps_host(rows.data(), n_points, dim, FPSX_LAYOUT_ROWS, n_samples, &s0, 1, pos.data(),
+  int rc = fps_host(cols.data(), n_points, dim, FPSX_LAYOUT_COLS, n_samples, &s0, 1, pos.data(),
                     kTieLowest, FLT_MAX);
   if (rc) return rc;
   for (size_t i = 0; i < n_samples; ++i) sampled_point_indices[i] = (size_t)perm[pos[i]];

@@ -25,6 +25,11 @@
  *   oracle_kdline_exhaustive equality with oracle_kdline_quickfps and the
  *                         golden vectors (the exactness argument of bucket
  *                         pruning, checked on tie-heavy lattices).
+ *   oracle_fps_f64,       (accuracy yardstick, double precision) brute force of
+ *   oracle_coverage_f64   the definitions on tiny clouds; the f64 run's pick
+ *                         distances equal the coverage radius of its prefixes;
+ *                         on integer lattices (every value exact in float and
+ *                         double) the f64 picks equal oracle_fps's.
  *   oracle_npdu           golden vectors of the reference _fps_npdu_sampling;
  *                         with a window covering the cloud it must equal
  *                         oracle_fps_ex(lowest) (the full update case).
@@ -185,6 +190,96 @@ ORACLE_API int oracle_npdu(const float *pts, size_t n, size_t d, size_t m, size_
     }
     free(dmin);
     return 0;
+}
+
+static inline double sqdist_f64(const float *pts, size_t d, size_t i, size_t p)
+{
+    const float *a = pts + i * d, *b = pts + p * d;
+    if (d == 3) {
+        double x = (double)a[0] - (double)b[0];
+        double y = (double)a[1] - (double)b[1];
+        double z = (double)a[2] - (double)b[2];
+        return (x * x + y * y) + z * z;
+    }
+    double acc = 0.0;
+    for (size_t j = 0; j < d; ++j) {
+        double diff = (double)a[j] - (double)b[j];
+        acc += diff * diff;
+    }
+    return acc;
+}
+
+/*
+ * Accuracy yardstick, not a restatement of the reference: the same greedy
+ * loop as src/lib.cpp:211-238 (start, update with `<`, argmax with `>=`),
+ * with every coordinate difference, square and sum in double.  Where the
+ * float32 loop and this one agree on every pick, the float32 picks are the
+ * exact farthest points; where they part, the coverage radius below measures
+ * how much it matters.  picked_dist (optional) gets the squared distance of
+ * each pick to the picks before it (+inf for pick 0).
+ */
+ORACLE_API int oracle_fps_f64(const float *pts, size_t n, size_t d, size_t m, size_t start,
+                              uint64_t *out, double *picked_dist)
+{
+    if (m == 0)
+        return 0;
+    double *dmin = (double *)malloc(n * sizeof(double));
+    if (!dmin)
+        return -1;
+    for (size_t i = 0; i < n; ++i)
+        dmin[i] = INFINITY;
+    size_t last = start;
+    out[0] = last;
+    if (picked_dist)
+        picked_dist[0] = INFINITY;
+    for (size_t s = 1; s < m; ++s) {
+        size_t best = 0;
+        double bestv = -1.0;
+        for (size_t i = 0; i < n; ++i) {
+            double acc = sqdist_f64(pts, d, i, last);
+            if (acc < dmin[i])
+                dmin[i] = acc;
+            if (dmin[i] >= bestv) {
+                bestv = dmin[i];
+                best = i;
+            }
+        }
+        last = best;
+        out[s] = last;
+        if (picked_dist)
+            picked_dist[s] = bestv;
+    }
+    free(dmin);
+    return 0;
+}
+
+/*
+ * Coverage radius of a pick set, in double: max over points of the Euclidean
+ * distance to the nearest pick (the quantity farthest point sampling
+ * greedily shrinks).  Returns -1.0 on allocation failure.
+ */
+ORACLE_API double oracle_coverage_f64(const float *pts, size_t n, size_t d, const uint64_t *picks,
+                                      size_t k)
+{
+    double *dmin = (double *)malloc(n * sizeof(double));
+    if (!dmin)
+        return -1.0;
+    for (size_t i = 0; i < n; ++i)
+        dmin[i] = INFINITY;
+    for (size_t s = 0; s < k; ++s) {
+        const size_t p = (size_t)picks[s];
+        for (size_t i = 0; i < n; ++i) {
+            double acc = sqdist_f64(pts, d, i, p);
+            if (acc < dmin[i])
+                dmin[i] = acc;
+        }
+    }
+    double r = 0.0;
+    for (size_t i = 0; i < n; ++i)
+        if (dmin[i] > r)
+            r = dmin[i];
+    free(dmin);
+    return sqrt(r);
 }
 
 /* fps_sampling / fps_sampling_multi_start_index semantics (src/lib.cpp:111-246). */

@@ -10,6 +10,9 @@ Two layers:
   ``kdline_exhaustive`` call the C restatement in ``fps_oracle.c`` (built into
   ``oracle/liboracle.so`` by ``oracle/Makefile``).  Each C function cites the
   reference file:line it restates.
+* ``fps_f64``, ``coverage_radius`` and ``accuracy_vs_f64`` are the
+  double-precision yardstick: index agreement and coverage-radius relative
+  error of a float32 pick sequence against the same greedy loop in double.
 * ``fps_numpy`` is an independent vectorised numpy restatement of
   src/lib.cpp:111-246 (float32 elementwise ops are unfused, round-to-nearest),
   used to cross-check the C code without sharing any of it.
@@ -59,6 +62,10 @@ def lib() -> ctypes.CDLL:
         L.oracle_fps.restype = ctypes.c_int
         L.oracle_kdline_perm.argtypes = [_f32p, sz, sz, sz, _u64p, _u64p, _u64p]
         L.oracle_kdline_perm.restype = ctypes.c_long
+        L.oracle_fps_f64.argtypes = [_f32p, sz, sz, sz, sz, _u64p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_fps_f64.restype = ctypes.c_int
+        L.oracle_coverage_f64.argtypes = [_f32p, sz, sz, _u64p, sz]
+        L.oracle_coverage_f64.restype = ctypes.c_double
         L.oracle_npdu.argtypes = [_f32p, sz, sz, sz, sz, sz, _u64p]
         L.oracle_npdu.restype = ctypes.c_int
         for name in ("oracle_kdline_quickfps", "oracle_kdline_exhaustive"):
@@ -149,6 +156,50 @@ def npdu(pc: np.ndarray, n_samples: int, w: int, start_idx: int = 0) -> np.ndarr
     if rc != 0:
         raise MemoryError("oracle_npdu failed")
     return out
+
+
+def fps_f64(pc: np.ndarray, n_samples: int, start_idx: int = 0):
+    """(picks, squared pick distances) of the greedy loop in double precision
+    (accuracy yardstick for the float32 result, not a reference restatement)."""
+    pts = _rows(pc)
+    out = np.zeros(n_samples, dtype=np.uint64)
+    dist = np.zeros(n_samples, dtype=np.float64)
+    rc = lib().oracle_fps_f64(pts.ctypes.data_as(_f32p), pts.shape[0], pts.shape[1], n_samples,
+                              int(start_idx), out.ctypes.data_as(_u64p),
+                              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if rc != 0:
+        raise MemoryError("oracle_fps_f64 failed")
+    return out, dist
+
+
+def coverage_radius(pc: np.ndarray, picks) -> float:
+    """max over points of the distance (double) to the nearest of `picks`."""
+    pts = _rows(pc)
+    p = np.ascontiguousarray(np.asarray(picks, dtype=np.uint64))
+    r = lib().oracle_coverage_f64(pts.ctypes.data_as(_f32p), pts.shape[0], pts.shape[1],
+                                  p.ctypes.data_as(_u64p), p.size)
+    if r < 0:
+        raise MemoryError("oracle_coverage_f64 failed")
+    return float(r)
+
+
+def accuracy_vs_f64(pc: np.ndarray, picks, start_idx: int = 0) -> dict:
+    """Index agreement of `picks` with the double-precision greedy run from the
+    same start, and the relative difference of their coverage radii."""
+    picks = np.asarray(picks, dtype=np.uint64)
+    k = picks.size
+    pts = _rows(pc)
+    # One more greedy step: the (k+1)-th pick's distance is the coverage
+    # radius of the first k picks.
+    ref, dist = fps_f64(pts, min(k + 1, pts.shape[0]), start_idx)
+    r_ref = float(np.sqrt(dist[k])) if k < pts.shape[0] else 0.0
+    ref = ref[:k]
+    r_got = coverage_radius(pts, picks)
+    agree = int(np.argmin(picks == ref)) if not np.all(picks == ref) else k
+    return {"picks": k, "index_agreement": float(np.mean(picks == ref)),
+            "identical_prefix": agree, "coverage_radius_f64": r_ref,
+            "coverage_radius": r_got,
+            "coverage_radius_rel_err": abs(r_got - r_ref) / r_ref if r_ref > 0 else 0.0}
 
 
 def fps_numpy(pc: np.ndarray, n_samples: int, start_idx=0, lowest: bool = False,

@@ -229,3 +229,33 @@ def test_npdu_wrapper_window_default_and_asserts(monkeypatch):
         F.fps_npdu_sampling(pc, 3, start_idx=1000)
     with pytest.raises(TypeError):
         F.fps_npdu_sampling(pc, 3, start_idx=[0, 1])   # `0 <= list` in the reference's assert
+
+
+def test_kdline_permutation_special_values_fuzz(capi):
+    """The permutation (columns, branch-free partition, threaded subtrees)
+    against the oracle's restatement of src/_ext/KDTreeBase.h:84-207 on
+    clouds with NaN, +-inf, -0, duplicates and all-NaN columns."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(3)
+    for trial in range(120):
+        n = int(rng.integers(1, 3000))
+        d = int(rng.integers(1, 9))
+        q = (rng.integers(0, 4, (n, d)) if trial % 3 == 0 else rng.standard_normal((n, d))).astype(np.float32)
+        if trial % 2 == 0 and n > 5:
+            q[rng.integers(0, n, 3), rng.integers(0, d, 3)] = np.nan
+        if trial % 5 == 0 and n > 5:
+            q[rng.integers(0, n, 2)] = np.inf
+            q[rng.integers(0, n, 2), 0] = -0.0
+        if trial % 14 == 0:
+            q[:, 0] = np.nan
+        h = int(rng.integers(1, 14))
+        p1, l1, h1 = capi.kdline_perm(q, h)
+        p2, l2, h2 = O.kdline_perm(q, h)
+        np.testing.assert_array_equal(p1, p2, err_msg=f"trial {trial} n={n} d={d} h={h}")
+        np.testing.assert_array_equal(l1, l2)
+        np.testing.assert_array_equal(h1, h2)
+    big = np.random.default_rng(9).standard_normal((300_000, 3)).astype(np.float32)
+    big[::1000] = np.nan
+    for h in (5, 11):  # large enough for the threaded top levels
+        np.testing.assert_array_equal(capi.kdline_perm(big, h)[0], O.kdline_perm(big, h)[0])

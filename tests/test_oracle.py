@@ -247,3 +247,46 @@ def test_npdu_against_reference_random(reference):
         pc = rng.standard_normal((n, d)).astype(np.float32)
         s = int(rng.integers(0, n))
         np.testing.assert_array_equal(O.npdu(pc, n // 2, w, s), reference.fps_npdu(pc, n // 2, w, s))
+
+
+# --- double-precision yardstick (accuracy report) -----------------------------
+
+def test_fps_f64_equals_f32_on_exact_lattices():
+    # Integer coordinates below 2^7: every difference, square and sum is exact
+    # in float and in double, so the double loop must reproduce the float
+    # oracle (itself pinned by the golden vectors) pick for pick.
+    rng = np.random.default_rng(21)
+    for d in (1, 2, 3, 5):
+        pc = rng.integers(0, 100, (3000, d)).astype(np.float32)
+        got, dist = O.fps_f64(pc, 700, 11)
+        np.testing.assert_array_equal(got, O.fps(pc, 700, 11))
+        assert np.isinf(dist[0]) and np.all(np.diff(dist[1:]) <= 0)
+
+
+def test_fps_f64_brute_force_tiny():
+    rng = np.random.default_rng(22)
+    for trial in range(20):
+        n = int(rng.integers(2, 40))
+        pc = rng.standard_normal((n, 3)).astype(np.float32)
+        m = int(rng.integers(1, n + 1))
+        x = pc.astype(np.float64)
+        picks = [0]
+        for _ in range(1, m):
+            dmin = np.min(((x[:, None, :] - x[None, picks, :]) ** 2).sum(-1), axis=1)
+            picks.append(int(np.flatnonzero(dmin == dmin.max())[-1]))
+        np.testing.assert_array_equal(O.fps_f64(pc, m, 0)[0], picks)
+
+
+def test_coverage_radius_definition_and_consistency():
+    rng = np.random.default_rng(23)
+    pc = rng.standard_normal((500, 3)).astype(np.float32)
+    picks = rng.choice(500, 37, replace=False).astype(np.uint64)
+    x = pc.astype(np.float64)
+    want = np.sqrt(((x[:, None, :] - x[None, picks.astype(np.int64), :]) ** 2).sum(-1).min(axis=1).max())
+    assert abs(O.coverage_radius(pc, picks) - want) <= 1e-12 * want
+    # The greedy run's next pick distance is the radius of its prefix.
+    ref, dist = O.fps_f64(pc, 60, 4)
+    for k in (1, 7, 59):
+        assert abs(O.coverage_radius(pc, ref[:k]) - np.sqrt(dist[k])) <= 1e-12
+    acc = O.accuracy_vs_f64(pc, O.fps(pc, 59, 4), 4)
+    assert acc["index_agreement"] == 1.0 and acc["coverage_radius_rel_err"] == 0.0
