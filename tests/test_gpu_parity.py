@@ -271,3 +271,31 @@ def test_pruned_auto_plan_and_bucket_order(gpu):
     rc, out = gpu.kdline(lat, 1500, 123, 7)
     assert rc == 0, gpu.last_error()
     np.testing.assert_array_equal(out, O.kdline_quickfps(lat, 1500, 7, 123))
+
+
+def test_bucket_entry_full_size_matches_oracle(gpu):
+    # 1 M points, h = 9: threaded KD-line permutation on the host, pruned GPU
+    # plan with lowest-position ties; the oracle's QuickFPS restatement decides.
+    import fpsample_amd as F
+
+    pc = c2_cloud()
+    m = 2048
+    t = O.kdline_quickfps(pc, m, 9, 5)
+    got = F.bucket_fps_kdline_sampling(pc, m, 9, start_idx=5)
+    np.testing.assert_array_equal(got, t)
+
+
+def test_accuracy_against_fp64_yardstick(gpu):
+    # Reported error of the float32 result: index agreement and coverage-radius
+    # relative error against the same greedy loop in double precision.
+    # Tolerance: coverage radius within 1e-5 relative (a late divergence of
+    # near-equal candidates may swap picks but barely moves the radius).
+    import bench
+
+    for name, pc, m in (("uniform4k", bench.gen_uniform_4096(), 1024),
+                        ("gauss1m_prefix", c2_cloud(), 256)):
+        out = run_fps(gpu, pc, m, 0)
+        acc = O.accuracy_vs_f64(pc, out, 0)
+        log(f"{name}: {acc}")
+        assert acc["coverage_radius_rel_err"] <= 1e-5, acc
+        assert acc["index_agreement"] >= 0.9, acc
