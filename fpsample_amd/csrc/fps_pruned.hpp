@@ -18,6 +18,12 @@ struct PrunedParams {
 
 const KernelEntry *pruned_table(size_t *count);
 
+// Streaming pruned kernel: blocks of kSBlock positions in spatial order,
+// kSMetaWords LDS words per block (box, cached candidate, list entry).
+constexpr int kSBlock = 1024;
+constexpr int kSMetaWords = 12;
+const KernelEntry *stream_pruned_table(size_t *count);
+
 // Device scratch of the spatial ordering (grow-only, owned by the caller).
 struct SpatialScratch;
 

@@ -176,6 +176,33 @@ static int plan_stream(Ctx &c, size_t n, size_t dim, int tie, int B, size_t want
   return FPSX_OK;
 }
 
+// Streaming pruned plan: 1024-thread workgroups; G chosen so that a
+// workgroup caches about 256 blocks (LDS metadata ~11 KB), bounded by
+// co-residency and the exchange limits.  want_g > 0 forces G.
+static int plan_stream_pruned(Ctx &c, size_t n, int tie, size_t want_g, Plan &p) {
+  size_t cnt;
+  const KernelEntry *t = stream_pruned_table(&cnt);
+  const KernelEntry *k = nullptr;
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].tie == tie) k = &t[i];
+  if (!k) return fail(FPSX_EINVAL, "no streaming pruned kernel");
+  const size_t nblocks = (n + kSBlock - 1) / kSBlock;
+  size_t g = want_g ? want_g : std::max<size_t>(1, (nblocks + 255) / 256);
+  g = std::min<size_t>(g, std::min<size_t>(nblocks, 256));
+  size_t nbl = (nblocks + g - 1) / g;
+  size_t shmem = (size_t)kSMetaWords * nbl * sizeof(int);
+  if (shmem > 96 * 1024) return fail(FPSX_ETOOBIG, "streaming pruned plan: too many blocks per workgroup");
+  const size_t gmax = (size_t)c.max_blocks(k, shmem);
+  if (g > gmax) return fail(FPSX_ETOOBIG, "streaming pruned plan: grid exceeds co-residency");
+  p.kind = PlanKind::StreamPruned;
+  p.k = k;
+  p.block = 1024;
+  p.grid = (uint32_t)g;
+  p.chunk = (uint32_t)(nbl * kSBlock);
+  p.shmem = shmem;
+  return FPSX_OK;
+}
+
 static int plan_resident_shape(Ctx &c, size_t n, const KernelEntry *k, Plan &p,
                                PlanKind kind = PlanKind::Resident) {
   const size_t per = (size_t)k->block * k->ppt;
@@ -214,6 +241,11 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
   if (env && *env) {
     std::string s(env);
     int a = 0, b = 0, g = 0;
+    if (s.rfind("spruned", 0) == 0) {
+      if (dim != 3) return fail(FPSX_EINVAL, "FPSX_PLAN: the streaming pruned kernel is 3-D only");
+      int got = std::sscanf(s.c_str(), "spruned:%d", &g);
+      return plan_stream_pruned(c, n, tie, got == 1 ? (size_t)g : 0, p);
+    }
     if (std::sscanf(s.c_str(), "pruned:%d:%d", &a, &b) == 2) {
       const KernelEntry *k = find_pruned((int)dim, a, b, tie);
       if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no pruned kernel for dim " +
@@ -283,6 +315,10 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
     p = bp;
     return FPSX_OK;
   }
+  // Beyond the register file: 3-D clouds stream only the blocks a pick can
+  // change; other dimensions stream everything.
+  if (dim == 3 && m >= 64 && !(np && np[0] == '1') && plan_stream_pruned(c, n, tie, 0, p) == FPSX_OK)
+    return FPSX_OK;
   return plan_stream(c, n, dim, tie, 1024, 0, false, p);
 }
 
@@ -466,13 +502,19 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
   }
   PrunedParams pa;
   void *arg = &a;
-  if (p.kind == PlanKind::Pruned) {
+  if (p.kind == PlanKind::Pruned || p.kind == PlanKind::StreamPruned) {
     HIP_TRY(c.sorted.reserve(n * 3 * sizeof(float)));
     HIP_TRY(c.perm.reserve(n * sizeof(uint32_t)));
     if (!c.spatial) c.spatial = new_spatial_scratch();
-    int rc = spatial_order(d_pts, sp, ld, (uint32_t)n, (uint32_t)(64 * p.k->ppt),
-                           (float *)c.sorted.p, (uint32_t *)c.perm.p, *c.spatial, st);
+    const uint32_t block = (p.kind == PlanKind::Pruned) ? (uint32_t)(64 * p.k->ppt) : (uint32_t)kSBlock;
+    int rc = spatial_order(d_pts, sp, ld, (uint32_t)n, block, (float *)c.sorted.p,
+                           (uint32_t *)c.perm.p, *c.spatial, st);
     if (rc) return rc;
+    if (p.kind == PlanKind::StreamPruned) {
+      // Min-distances by position; the first pass writes every one.
+      HIP_TRY(c.dmin.reserve(n * sizeof(float)));
+      a.dmin = (float *)c.dmin.p;
+    }
     pa.fps = a;
     pa.sorted = (const float *)c.sorted.p;
     pa.sorted_ld = n;

@@ -299,3 +299,61 @@ def test_accuracy_against_fp64_yardstick(gpu):
         log(f"{name}: {acc}")
         assert acc["coverage_radius_rel_err"] <= 1e-5, acc
         assert acc["index_agreement"] >= 0.9, acc
+
+
+SPRUNED_PLANS = ["spruned", "spruned:1", "spruned:3", "spruned:16"]
+
+
+@pytest.mark.parametrize("plan", SPRUNED_PLANS)
+def test_stream_pruned_plans(gpu, plan_env, plan):
+    # Streaming kernel with per-block pruning: points in HBM in spatial order,
+    # only blocks a pick can change are read.  Both tie rules, forced prefix.
+    from fpsample_amd import device as D
+
+    plan_env(plan)
+    rng = np.random.default_rng(41)
+    clouds = [("gauss", c2_cloud(60000, seed=7)),
+              ("lattice_dups", np.repeat(rng.integers(0, 12, (9000, 3)), 3, axis=0).astype(np.float32)),
+              ("ragged_tail", rng.random((17 * 1024 + 5, 3)).astype(np.float32))]
+    for name, pc in clouds:
+        n = len(pc)
+        log(f"spruned {plan} {name} n={n} plan={gpu.plan_info(n, 3, m=700)}")
+        np.testing.assert_array_equal(run_fps(gpu, pc, 700, 3), O.fps(pc, 700, 3), err_msg=name)
+        np.testing.assert_array_equal(run_fps(gpu, pc, 700, [5, 9, n - 1]), O.fps(pc, 700, [5, 9, n - 1]))
+        soa = D.DeviceArray.from_numpy(np.ascontiguousarray(pc.T))
+        ds = D.DeviceArray.from_numpy(np.array([11], np.uint64))
+        out = D.DeviceArray((500,), np.uint64)
+        D.fps_sampling_device(soa, n, 3, 500, ds, 1, out, tie=D.TIE_LOWEST,
+                              init=float(np.finfo(np.float32).max))
+        D.synchronize()
+        D.device_status()
+        want = O.fps_ex(pc, 500, 11, lowest=True, init=float(np.finfo(np.float32).max))
+        np.testing.assert_array_equal(out.numpy(), want, err_msg=f"{name} lowest")
+
+
+def test_stream_pruned_special_values(gpu, plan_env):
+    plan_env("spruned:4")
+    rng = np.random.default_rng(42)
+    pc = (rng.random((40000, 3)) * 4e19).astype(np.float32)
+    pc[[5, 7000, 39999]] = np.nan
+    pc[9] = np.inf
+    pc[10, 1] = -np.inf
+    np.testing.assert_array_equal(run_fps(gpu, pc, 600, 0), O.fps(pc, 600, 0))
+    tiny = (rng.random((30000, 3)) * 1e-21).astype(np.float32)
+    np.testing.assert_array_equal(run_fps(gpu, tiny, 600, 0), O.fps(tiny, 600, 0))
+
+
+def test_lattice_streaming_kernels_agree(gpu, plan_env):
+    # 16.7 M lattice points with 25 % duplicates, 4096 picks: the pruned
+    # streaming kernel (auto plan) and the plain streaming kernel must give the
+    # same sequence; the oracle checks the prefix.
+    pc = c5_cloud()
+    plan_env(None)
+    info = gpu.plan_info(len(pc), 3, m=4096)
+    log(f"C5 auto plan {info}")
+    assert info["kind"] == 4
+    a = run_fps(gpu, pc, 4096, 0)
+    plan_env("stream:1024")
+    b = run_fps(gpu, pc, 4096, 0)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a[:24], O.fps(pc, 24, 0))
