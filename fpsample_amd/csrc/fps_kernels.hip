@@ -360,22 +360,25 @@ const KernelEntry *stream_table(size_t *count) {
   return kStream;
 }
 
-void launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t st) {
-  if (n == 0) return;
+hipError_t launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
   unsigned blocks = (unsigned)min((size_t)1024, (n + 255) / 256);
   hipLaunchKernelGGL(fill_u64, dim3(blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
 }
-void launch_fill_f32(float *p, size_t n, float v, hipStream_t st) {
-  if (n == 0) return;
+hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
   unsigned blocks = (unsigned)min((size_t)2048, (n + 255) / 256);
   hipLaunchKernelGGL(fill_f32, dim3(blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
 }
-void launch_strided_to_soa(const float *src, uint64_t sp, uint64_t sd, float *soa, uint32_t n,
-                           uint32_t dim, uint64_t ld, hipStream_t st) {
+hipError_t launch_strided_to_soa(const float *src, uint64_t sp, uint64_t sd, float *soa,
+                                 uint32_t n, uint32_t dim, uint64_t ld, hipStream_t st) {
   size_t total = (size_t)n * dim;
-  if (total == 0) return;
+  if (total == 0) return hipSuccess;
   unsigned blocks = (unsigned)min((size_t)4096, (total + 255) / 256);
   hipLaunchKernelGGL(strided_to_soa, dim3(blocks), dim3(256), 0, st, src, sp, sd, soa, n, dim, ld);
+  return hipGetLastError();
 }
 
 }  // namespace fpsx

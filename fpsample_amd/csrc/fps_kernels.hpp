@@ -47,11 +47,12 @@ struct KernelEntry {
 const KernelEntry *resident_table(size_t *count);
 const KernelEntry *stream_table(size_t *count);
 
-void launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t st);
-void launch_fill_f32(float *p, size_t n, float v, hipStream_t st);
+// Enqueue helpers; each returns the launch status.
+hipError_t launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t st);
+hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t st);
 // Copy a strided point array (coordinate j of point i at src[i*sp + j*sd])
 // into SoA layout (soa[j*ld + i]).
-void launch_strided_to_soa(const float *src, uint64_t sp, uint64_t sd, float *soa, uint32_t n,
-                           uint32_t dim, uint64_t ld, hipStream_t st);
+hipError_t launch_strided_to_soa(const float *src, uint64_t sp, uint64_t sd, float *soa,
+                                 uint32_t n, uint32_t dim, uint64_t ld, hipStream_t st);
 
 }  // namespace fpsx

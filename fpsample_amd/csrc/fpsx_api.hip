@@ -425,8 +425,8 @@ static int common_params(Ctx &c, const float *d_pts, size_t sp, size_t ld, size_
   const size_t xwords = xchg_words(reps);
   HIP_TRY(c.xchg.reserve(xwords * sizeof(uint64_t)));
   HIP_TRY(c.status.reserve(sizeof(uint64_t)));
-  launch_fill_u64((uint64_t *)c.xchg.p, xwords, 0, st);
-  launch_fill_u64((uint64_t *)c.status.p, 1, 0, st);
+  HIP_TRY(launch_fill_u64((uint64_t *)c.xchg.p, xwords, 0, st));
+  HIP_TRY(launch_fill_u64((uint64_t *)c.status.p, 1, 0, st));
   std::memset(&a, 0, sizeof(a));
   a.pts = d_pts;
   a.ld = ld;
@@ -486,7 +486,7 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
   if (p.kind == PlanKind::Stream && sp != 1) {
     // The streaming kernel re-reads every point on every pass: give it SoA.
     HIP_TRY(c.pts.reserve(n * dim * sizeof(float)));
-    launch_strided_to_soa(d_pts, sp, ld, (float *)c.pts.p, (uint32_t)n, (uint32_t)dim, n, st);
+    HIP_TRY(launch_strided_to_soa(d_pts, sp, ld, (float *)c.pts.p, (uint32_t)n, (uint32_t)dim, n, st));
     d_pts = (const float *)c.pts.p;
     sp = 1;
     ld = n;
@@ -498,7 +498,7 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
   if (p.kind == PlanKind::Stream) {
     HIP_TRY(c.dmin.reserve(n * sizeof(float)));
     a.dmin = (float *)c.dmin.p;
-    launch_fill_f32(a.dmin, n, init, st);
+    HIP_TRY(launch_fill_f32(a.dmin, n, init, st));
   }
   PrunedParams pa;
   void *arg = &a;
@@ -567,8 +567,8 @@ int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
   } else {
     HIP_TRY(c.rows.reserve(bytes));
     HIP_TRY(hipMemcpyAsync(c.rows.p, points, bytes, hipMemcpyHostToDevice, st));
-    launch_strided_to_soa((const float *)c.rows.p, dim, 1, (float *)c.pts.p, (uint32_t)n,
-                          (uint32_t)dim, n, st);
+    HIP_TRY(launch_strided_to_soa((const float *)c.rows.p, dim, 1, (float *)c.pts.p, (uint32_t)n,
+                                  (uint32_t)dim, n, st));
   }
   HIP_TRY(hipMemcpyAsync(c.starts.p, starts, n_starts * sizeof(uint64_t), hipMemcpyHostToDevice,
                          st));
@@ -788,7 +788,7 @@ FPSX_EXPORT int fpsx_autotune(const float *d_points, size_t stride_point, size_t
   const size_t m = std::max<size_t>(2, std::min(passes ? passes : 256, n_points));
   HIP_TRY(c.starts.reserve(sizeof(uint64_t)));
   HIP_TRY(c.out.reserve(m * sizeof(uint64_t)));
-  launch_fill_u64((uint64_t *)c.starts.p, 1, 0, st);
+  HIP_TRY(launch_fill_u64((uint64_t *)c.starts.p, 1, 0, st));
   std::vector<Plan> cands = candidates(c, n_points, dim, tie);
   if (cands.empty()) return fail(FPSX_EINVAL, "no feasible plan");
   hipEvent_t e0, e1;

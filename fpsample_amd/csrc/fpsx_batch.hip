@@ -132,9 +132,9 @@ int BatchRun::prepare(Ctx &c, const KernelEntry *kernel, const std::vector<Batch
 }
 
 int BatchRun::enqueue(hipStream_t st) {
-  launch_fill_u64((uint64_t *)d_counters.p, counter_words, 0, st);
-  launch_fill_u64((uint64_t *)d_xchg.p, xchg_words, 0, st);
-  launch_fill_u64((uint64_t *)d_status.p, 1, 0, st);
+  HIP_TRY(launch_fill_u64((uint64_t *)d_counters.p, counter_words, 0, st));
+  HIP_TRY(launch_fill_u64((uint64_t *)d_xchg.p, xchg_words, 0, st));
+  HIP_TRY(launch_fill_u64((uint64_t *)d_status.p, 1, 0, st));
   void *args[] = {&params};
   HIP_TRY(hipLaunchKernel(k->fn, dim3(grid), dim3(k->block), args, 0, st));
   return FPSX_OK;
@@ -380,8 +380,8 @@ FPSX_EXPORT int fpsx_fps_batch(const float *points, size_t dim, const uint64_t *
     std::lock_guard<std::mutex> lk(cp->mu);
     HIP_TRY(hipMemcpyAsync(rows.p, points, total * dim * sizeof(float), hipMemcpyHostToDevice,
                            cp->stream));
-    launch_strided_to_soa((const float *)rows.p, dim, 1, (float *)soa.p, (uint32_t)total,
-                          (uint32_t)dim, total, cp->stream);
+    HIP_TRY(launch_strided_to_soa((const float *)rows.p, dim, 1, (float *)soa.p, (uint32_t)total,
+                                  (uint32_t)dim, total, cp->stream));
     HIP_TRY(hipStreamSynchronize(cp->stream));
   }
   fpsx_batch *b = nullptr;

@@ -288,6 +288,8 @@ void fpsx_server::loop() {
       for (auto &r : active) {
         r->done = true;
         r->error = rc;
+        if (r->uploaded) hipEventDestroy(r->uploaded);
+        r->uploaded = nullptr;
       }
       for (auto &r : pending) {
         r->done = true;
@@ -327,7 +329,12 @@ FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passe
   s->arena.reset(new Arena(s->budget));
   e = hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking);
-  if (e != hipSuccess) return fail(FPSX_EHIP, "streams: " + std::string(hipGetErrorString(e)));
+  if (e != hipSuccess) {
+    const std::string msg = "streams: " + std::string(hipGetErrorString(e));
+    if (s->compute) hipStreamDestroy(s->compute);
+    hipFree(s->arena_base);
+    return fail(FPSX_EHIP, msg);
+  }
   fpsx_server *raw = s.get();
   raw->thread = std::thread([raw] { raw->loop(); });
   *out = s.release();

@@ -178,20 +178,25 @@ int spatial_order(const float *pts, uint64_t sp, uint64_t ld, uint32_t n, uint32
   const unsigned nb = (unsigned)std::min<uint64_t>(kBoxBlocks, (n + kBoxThreads - 1) / kBoxThreads);
   hipLaunchKernelGGL(bbox_partial, dim3(nb), dim3(kBoxThreads), 0, st, pts, sp, ld, n,
                      (float *)S.part.p);
+  HIP_OR_FAIL(hipGetLastError());
   hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBoxThreads), 0, st, (const float *)S.part.p, nb,
                      (float *)S.box.p);
+  HIP_OR_FAIL(hipGetLastError());
   const unsigned g = (n + 255) / 256;
   hipLaunchKernelGGL(morton_codes, dim3(g), dim3(256), 0, st, pts, sp, ld, n,
                      (const float *)S.box.p, (uint32_t *)S.keys.p, (uint32_t *)S.vals.p);
+  HIP_OR_FAIL(hipGetLastError());
   HIP_OR_FAIL(hipcub::DeviceRadixSort::SortPairs(S.tmp.p, t1, (uint32_t *)S.keys.p,
                                                  (uint32_t *)S.keys_alt.p, (uint32_t *)S.vals.p,
                                                  (uint32_t *)S.vals_alt.p, (int)n, 0, 30, st));
   hipLaunchKernelGGL(block_keys, dim3(g), dim3(256), 0, st, (const uint32_t *)S.vals_alt.p, n,
                      block, (uint64_t *)S.keys64.p);
+  HIP_OR_FAIL(hipGetLastError());
   HIP_OR_FAIL(hipcub::DeviceRadixSort::SortKeys(S.tmp.p, t2, (uint64_t *)S.keys64.p,
                                                 (uint64_t *)S.keys64_alt.p, (int)n, 0, hi_bit, st));
   hipLaunchKernelGGL(gather_sorted, dim3(g), dim3(256), 0, st, pts, sp, ld, n,
                      (const uint64_t *)S.keys64_alt.p, sorted, perm);
+  HIP_OR_FAIL(hipGetLastError());
   return FPSX_OK;
 }
 
