@@ -60,8 +60,12 @@ int fpsx_device_status(void *stream);
 /*
  * The launch the planner picks for a cloud of n_points x dim and n_samples
  * picks under `tie`: kind 0 = register-resident, 1 = HBM-streaming,
- * 2 = register-resident with spatial order and per-wave pruning; threads per
- * workgroup, points per thread, workgroups, points per workgroup.
+ * 2 = register-resident with spatial order and per-wave pruning,
+ * 4 = HBM-streaming with spatial order and per-block pruning; threads per
+ * workgroup, points per thread (0 when streaming), workgroups, points per
+ * workgroup.  FPSX_PLAN in the environment forces a plan: "resident:B:PPT",
+ * "pruned:B:PPT", "stream:B[:G]", "generic:B[:G]", "spruned[:G]";
+ * FPSX_NO_PRUNE=1 keeps the planner off the pruned kinds.
  */
 int fpsx_plan_info(size_t n_points, size_t dim, int tie, size_t n_samples, int *kind, int *block,
                    int *ppt, int *grid, int *chunk);
