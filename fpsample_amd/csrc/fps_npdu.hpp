@@ -17,4 +17,9 @@ struct NpduParams {
 // coordinates read from memory for the window only); tie is always lowest.
 const KernelEntry *npdu_table(size_t *count);
 
+// Streaming form (any dimension, ppt 0): blocks of kNpduBlock indices; LDS
+// holds dim floats of pivot plus 2 words per block of the workgroup.
+constexpr int kNpduBlock = 1024;
+const KernelEntry *npdu_stream_kernel();
+
 }  // namespace fpsx

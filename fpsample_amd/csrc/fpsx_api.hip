@@ -337,8 +337,36 @@ static const KernelEntry *find_npdu(int dim, int B, int ppt) {
 // suffices (16384 / 32768 points); beyond, 1024-thread workgroups with the
 // share per thread that minimises 0.012 us per slot + 0.0125 us per
 // workgroup of exchange (fitted to MI355X sweeps at 50 K and 100 K points).
+// Streaming windowed-update plan (any dimension): G chosen so that a
+// workgroup caches about 256 blocks; want_g > 0 forces G.
+static int plan_npdu_stream(Ctx &c, size_t n, size_t dim, size_t want_g, Plan &p) {
+  const KernelEntry *k = npdu_stream_kernel();
+  const size_t nblocks = (n + kNpduBlock - 1) / kNpduBlock;
+  size_t g = want_g ? want_g : std::max<size_t>(1, (nblocks + 255) / 256);
+  g = std::min<size_t>(g, std::min<size_t>(nblocks, 256));
+  const size_t nbl = (nblocks + g - 1) / g;
+  const size_t shmem = dim * sizeof(float) + 2 * nbl * sizeof(int);
+  if (shmem > 96 * 1024)
+    return fail(FPSX_ETOOBIG, "windowed-update streaming plan: too many blocks per workgroup");
+  const size_t gmax = (size_t)c.max_blocks(k, shmem);
+  if (g > gmax)
+    return fail(FPSX_ETOOBIG, "windowed-update streaming plan: grid exceeds co-residency");
+  p.kind = PlanKind::Npdu;
+  p.k = k;
+  p.block = 1024;
+  p.grid = (uint32_t)g;
+  p.chunk = (uint32_t)(nbl * kNpduBlock);
+  p.shmem = shmem;
+  return FPSX_OK;
+}
+
 int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p) {
   const char *env = std::getenv("FPSX_NPDU_PLAN");
+  if (env && std::strncmp(env, "stream", 6) == 0) {
+    int g = 0;
+    const int got = std::sscanf(env, "stream:%d", &g);
+    return plan_npdu_stream(c, n, dim, got == 1 ? (size_t)g : 0, p);
+  }
   if (env && *env) {
     int d = 0, b = 0, q = 0;
     if (std::sscanf(env, "%d:%d:%d", &d, &b, &q) != 3 || (d != 0 && d != (int)dim))
@@ -382,8 +410,8 @@ int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p) {
     const KernelEntry *k = find_npdu(0, 1024, 32);
     if (k && plan_resident_shape(c, n, k, p, PlanKind::Npdu) == FPSX_OK) return FPSX_OK;
   }
-  return fail(FPSX_ETOOBIG, "cloud of " + std::to_string(n) +
-                                " points exceeds the windowed-update kernel's co-resident grid");
+  // Beyond the register file: min-distances in HBM, window blocks only.
+  return plan_npdu_stream(c, n, dim, 0, p);
 }
 
 // Candidate plans the tuner times for a cloud: every feasible resident shape
@@ -536,6 +564,11 @@ int launch_npdu(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld,
   // Any window of 2n or more covers the whole cloud; the clamp keeps the
   // kernel's index arithmetic far from overflow.
   a.window = std::min<uint64_t>((uint64_t)window, (uint64_t)4 * n + 4);
+  if (p.k->ppt == 0) {
+    // Streaming form: min-distances by index; the start pass writes them all.
+    HIP_TRY(c.dmin.reserve(n * sizeof(float)));
+    a.fps.dmin = (float *)c.dmin.p;
+  }
   return dispatch(c, p, &a, st);
 }
 

@@ -170,7 +170,9 @@ int fpsx_fps_npdu_sampling_device(const float *d_points, size_t stride_point, si
 /*
  * Kernel the windowed-update entries would use for a cloud: kernel_dim 3
  * (coordinates in registers) or 0 (any dimension), workgroup size, points per
- * thread and grid.  FPSX_NPDU_PLAN="D:B:PPT" in the environment forces a shape.
+ * thread and grid.  FPSX_NPDU_PLAN="D:B:PPT" in the environment forces a shape,
+ * "stream[:G]" the streaming form (any dimension, min-distances in HBM, used
+ * automatically beyond the register file; reported as kernel_dim 0, ppt 0).
  */
 int fpsx_npdu_plan_info(size_t n_points, size_t dim, int *kernel_dim, int *block, int *ppt,
                         int *grid);

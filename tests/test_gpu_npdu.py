@@ -148,3 +148,31 @@ def test_npdu_error_codes(gpu):
     assert gpu.npdu(pc, 3, 4, 10)[0] == 2      # start out of range
     assert gpu.npdu(pc, 11, 4, 0)[0] == 3      # n_samples > n_points
     assert gpu.npdu(pc, 3, 4, 0, dim=0)[0] == 1
+
+
+@pytest.mark.parametrize("plan", ["stream", "stream:1", "stream:4", "stream:13"])
+def test_npdu_streaming_form(gpu, npdu_plan, plan):
+    # Min-distances in HBM, per-block cached bests, window blocks only.
+    npdu_plan(plan)
+    rng = np.random.default_rng(47)
+    for name, pc, w in (("uniform3", rng.random((20000, 3)).astype(np.float32), 40),
+                        ("dims5", rng.standard_normal((15000, 5)).astype(np.float32), 3000),
+                        ("lattice_dups", np.repeat(rng.integers(0, 6, (4000, 2)), 4, axis=0).astype(np.float32), 17),
+                        ("tiny_window", np.sort(rng.random((13 * 1024 + 3, 3)).astype(np.float32), axis=0), 0)):
+        log(f"npdu stream {plan} {name} plan={gpu.npdu_plan_info(len(pc), pc.shape[1])}")
+        for s0 in (0, len(pc) - 1, 5000):
+            np.testing.assert_array_equal(run(gpu, pc, 400, w, s0), O.npdu(pc, 400, w, s0),
+                                          err_msg=f"{plan} {name} s={s0}")
+
+
+def test_npdu_beyond_register_file(gpu):
+    # 9 M points: more than any register-resident shape holds, so the planner
+    # takes the streaming form; the oracle checks a prefix.
+    rng = np.random.default_rng(48)
+    n = 9 * (1 << 20)
+    pc = rng.random((n, 3)).astype(np.float32)
+    info = gpu.npdu_plan_info(n, 3)
+    log(f"npdu 9M plan {info}")
+    assert info["ppt"] == 0
+    w = int(n / 4096 * 16)
+    np.testing.assert_array_equal(run(gpu, pc, 40, w, 123), O.npdu(pc, 40, w, 123))
