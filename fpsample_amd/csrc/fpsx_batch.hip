@@ -152,7 +152,7 @@ struct fpsx_batch {
   int dim = 0;
   size_t n_clouds = 0;
   BatchRun run;
-  bool fallback = false;  // dimension without a batch kernel: one launch per cloud
+  bool fallback = false;  // no batch kernel for this dimension or cloud size: one launch per cloud
   std::vector<JobSpec> specs;
   const float *d_pts = nullptr;
   uint64_t ld = 0;
@@ -278,6 +278,14 @@ FPSX_EXPORT int fpsx_batch_create(const float *d_points, size_t ld, size_t dim,
     if (!chosen) chosen = cands[0];
   }
   rc = batch_prepare_kernel(b.get(), chosen);
+  if (rc == FPSX_ETOOBIG) {
+    // Some cloud needs more workgroups than a team may have: every cloud
+    // gets a planned launch of its own (the single-cloud kernels have no
+    // such limit).
+    b->fallback = true;
+    *out = b.release();
+    return FPSX_OK;
+  }
   if (rc) return rc;
   if (flags & FPSX_BATCH_GRAPH) {
     HIP_TRY(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));

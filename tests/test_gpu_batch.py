@@ -91,3 +91,20 @@ def test_batch_graph_replays(gpu):
         got1, _ = D.fps_sampling_tensor(D.DeviceArray.from_numpy(rc_pc), ms[rep], 0)
         np.testing.assert_array_equal(got1.numpy(), O.fps(rc_pc, ms[rep], 0))
     b.close()
+
+
+def test_batch_oversize_cloud_falls_back(gpu):
+    # A cloud larger than any batch team (64 workgroups) sends the batch to
+    # one planned launch per cloud; every result still equals the oracle's.
+    import fpsample_amd as F
+    from fpsample_amd import batch as Bt
+
+    rng = np.random.default_rng(77)
+    big = rng.random((1_200_000, 3)).astype(np.float32)
+    clouds = [rng.random((5000, 3)).astype(np.float32), big, rng.random((300, 3)).astype(np.float32)]
+    ms = [500, 64, 100]
+    got = F.fps_sampling_batch(clouds, ms, 2)
+    for g, w in zip(got, oracle_all(clouds, ms, [2, 2, 2])):
+        np.testing.assert_array_equal(g, w)
+    b = Bt.DeviceBatch(clouds, ms, [2, 2, 2])
+    assert b.info()["block"] == 0  # the per-cloud form
