@@ -53,6 +53,10 @@ __device__ __forceinline__ bool run_job(const BatchParams &a, const BatchJob &J,
   float pv[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) pv[d] = pts[(size_t)d * J.ld + pidx];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) settle<D>(c[k]);
+  settle<PPT>(dm);
+  settle<D>(pv);
   if (J.s_begin == 0 && r == 0 && tid == B - 64) a.out[J.out_off] = pidx;
 
   uint64_t *area = a.xchg + (size_t)J.area * kTeamXchgWords;

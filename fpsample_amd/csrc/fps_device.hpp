@@ -277,6 +277,19 @@ __device__ __forceinline__ void record_pick_dyn(const FpsParams &a, uint32_t s, 
     for (uint32_t j = lane; j < dim; j += 64) a.out_pts[(size_t)s * dim + j] = pv[j];
 }
 
+// Marks registers filled by global loads as settled.  The compiler's wait
+// counters are merged over a loop's entry and back edge, so a register still
+// awaiting a load when a pass loop is entered makes the loop header wait for
+// every outstanding memory operation on every pass -- including the store
+// that recorded the previous pick, a full memory round trip on the critical
+// path of the recording wave.  An empty asm that reads and rewrites the
+// register moves that wait to this point, once.
+template <int N>
+__device__ __forceinline__ void settle(float (&v)[N]) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) asm volatile("" : "+v"(v[j]));
+}
+
 // Coordinates of a forced pick (multi-start prefix).  An index outside the
 // cloud is reported through the status word and replaced by 0, so no access
 // ever leaves the arrays.
@@ -289,6 +302,7 @@ __device__ __forceinline__ uint32_t load_forced(const FpsParams &a, uint32_t s, 
   }
 #pragma unroll
   for (int j = 0; j < D; ++j) pv[j] = a.pts[(size_t)idx * a.sp + (size_t)j * a.ld];
+  settle<D>(pv);
   return (uint32_t)idx;
 }
 

@@ -64,6 +64,8 @@ __global__ void __launch_bounds__(B) fps_resident(FpsParams a) {
     for (int j = 0; j < D; ++j) c[k][j] = live ? a.pts[(size_t)i * a.sp + (size_t)j * a.ld] : 0.0f;
     dm[k] = live ? a.init : -1.0f;
   }
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) settle<D>(c[k]);
 
   float pv[D];
   uint32_t pidx = load_forced<D>(a, 0, pv);

@@ -63,6 +63,11 @@ __global__ void __launch_bounds__(B) fps_npdu(NpduParams a) {
 #pragma unroll
   for (int j = 0; j < DC; ++j)
     pv[j] = (D > 0) ? f.pts[(size_t)pidx * f.sp + (size_t)j * f.ld] : 0.0f;
+  if constexpr (D > 0) {
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) settle<DC>(c[k]);
+    settle<DC>(pv);
+  }
   const bool recorder = (g == 0 && tid >= B - 64);
   auto record = [&](uint32_t s) {
     if constexpr (D > 0) {

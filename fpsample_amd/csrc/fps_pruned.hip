@@ -95,17 +95,19 @@ __global__ void __launch_bounds__(B) fps_pruned(PrunedParams a) {
     lo[j] = INFINITY;
     hi[j] = -INFINITY;
   }
+  // Unconditional loads of a clamped position (a load under a branch would
+  // wait for each one in turn); padding slots are then overwritten.
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
     const uint32_t i = wbase + (uint32_t)(k * 64 + lane);
     const bool live = i < f.n;
+    const uint32_t ic = live ? i : f.n - 1u;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      c[k][j] = live ? a.sorted[(size_t)j * a.sorted_ld + i] : 0.0f;
-      if (live) {
-        lo[j] = fminf(lo[j], c[k][j]);
-        hi[j] = fmaxf(hi[j], c[k][j]);
-      }
+      const float v = a.sorted[(size_t)j * a.sorted_ld + ic];
+      c[k][j] = live ? v : 0.0f;
+      lo[j] = live ? fminf(lo[j], v) : lo[j];
+      hi[j] = live ? fmaxf(hi[j], v) : hi[j];
     }
     dm[k] = live ? f.init : -1.0f;
   }
@@ -195,6 +197,446 @@ __global__ void __launch_bounds__(B) fps_pruned(PrunedParams a) {
       for (int j = 0; j < D; ++j) pv[j] = __int_as_float(win[2 + j]);
     }
     if (g == 0 && tid >= B - 64) record_pick<D>(f, s, pidx, pv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Paired form: one cross-workgroup exchange settles up to two picks.
+//
+// Every workgroup publishes two candidates: its best point A (as above) and
+// its lookahead A' -- its best point after a hypothetical update with A as
+// the pivot, computed without storing anything.  From the G pairs every
+// workgroup derives, identically:
+//   p1 = the best A (workgroup w1), the next pick exactly as before;
+//   the state of each workgroup after p1:
+//     w1: its best is A'_w1 exactly (its lookahead pivot was p1 itself);
+//     w != w1 whose A keeps its value under p1 (min(dmin, dist(A, p1)) ==
+//       dmin, the kernel's own arithmetic): its best is still A_w, since
+//       every other point of w ranked below A_w and can only go down;
+//     any other w: unknown, but every point of w now ranks strictly below
+//       A_w (its pre-update key);
+//   C = the best of the known bests.  When C ranks above A_w of every
+//   unknown workgroup, nothing can beat it: p2 = C, the pick after p1.
+//   Otherwise the round yields p1 alone.
+// Both picks are then applied (box-pruned) before the next pair is formed.
+// The picks are those of the one-pick loop; only the number of exchanges
+// changes.  A workgroup none of whose waves changed keeps A and A' from
+// the previous round.
+// Only w1's lookahead is ever used, so a workgroup computes it only when it
+// could be w1.  The workgroups exchange their bounding boxes once; after each
+// round every workgroup finds, from the picks and the published A's, the
+// workgroups that certainly keep their A (the picks lie at least as far from
+// their box as their A's value) and the best such A among the others, M.
+// In the next round a workgroup whose A does not beat M cannot be w1 and
+// skips the lookahead (its published A' is then stale and marked so; it is
+// computed again before it can be used).  The grid must satisfy 2 <= G <= 64
+// (one polling wave evaluates all pairs).
+// ---------------------------------------------------------------------------
+template <int D, int B, int PPT, int TIE>
+__global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
+  constexpr int W = B / 64;
+  constexpr int NW = 2 + D;      // value, original index, coordinates
+  constexpr int NP = 2 * NW;     // words published per workgroup: A, then A'
+  constexpr int PER_WG = B * PPT;
+  __shared__ int s_top[2][NW + 1][W];  // wave bests + "changed" flags, by exchange parity
+  __shared__ int s_la[2][NW][W];       // wave lookahead bests
+  __shared__ int s_res[2][NP + 4];     // p1, p2, two-pick flag, all-arrived flag, M
+  __shared__ float s_gbox[2 * D][64];  // every workgroup's bounding box
+  __shared__ float s_wbox[2 * D][W];   // this workgroup's wave boxes
+  __shared__ int s_ok;
+  __shared__ uint32_t s_perm[PER_WG];
+
+  const FpsParams &f = a.fps;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const uint32_t G = gridDim.x;
+  const uint32_t g = blockIdx.x;
+  const uint32_t base = g * (uint32_t)PER_WG;
+  const uint32_t wbase = base + (uint32_t)(wave * 64 * PPT);
+  const bool recorder = (g == 0 && tid >= B - 64);
+
+  for (uint32_t q = tid; q < PER_WG; q += B)
+    s_perm[q] = (base + q < f.n) ? a.perm[base + q] : 0u;
+
+  float c[PPT][D];
+  float dm[PPT];
+  float lo[D], hi[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    lo[j] = INFINITY;
+    hi[j] = -INFINITY;
+  }
+  // Unconditional loads of a clamped position (a load under a branch would
+  // wait for each one in turn); padding slots are then overwritten.
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint32_t i = wbase + (uint32_t)(k * 64 + lane);
+    const bool live = i < f.n;
+    const uint32_t ic = live ? i : f.n - 1u;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const float v = a.sorted[(size_t)j * a.sorted_ld + ic];
+      c[k][j] = live ? v : 0.0f;
+      lo[j] = live ? fminf(lo[j], v) : lo[j];
+      hi[j] = live ? fmaxf(hi[j], v) : hi[j];
+    }
+    dm[k] = live ? f.init : -1.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    lo[j] = wave_min_f(lo[j]);
+    hi[j] = wave_max_f(hi[j]);
+  }
+  lds_barrier();  // s_perm complete
+
+  // Best (value, original index, coordinates) of this wave over val(k).
+  // Each lane carries its best slot's coordinates through the scan (static
+  // indices only: selecting c[slot] afterwards by a uniform slot number is
+  // turned by the compiler into a dynamic index, which puts c in scratch).
+  auto wave_best = [&](auto val, int &ov, int &oi, float(&oc)[D]) {
+    int bv = INT_MIN;
+    int bk = 0;
+    float bc[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) bc[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int b = __float_as_int(val(k));
+      const bool take = (TIE == kTieHighest) ? (b >= bv) : (b > bv);
+      bv = take ? b : bv;
+      bk = take ? k : bk;
+#pragma unroll
+      for (int j = 0; j < D; ++j) bc[j] = take ? c[k][j] : bc[j];
+    }
+    const uint32_t bpos = (uint32_t)(bk * 64 + lane);
+    int wv;
+    const int wl = wave_argmax_lane<TIE>(bv, bpos, wv);
+    const uint32_t wpos = (uint32_t)readlane_i((int)bpos, wl);
+#pragma unroll
+    for (int j = 0; j < D; ++j) oc[j] = readlane_f(bc[j], wl);
+    ov = wv;
+    oi = (int)s_perm[wave * 64 * PPT + wpos];
+  };
+
+  float p1[D], p2[D];
+  uint32_t i1 = load_forced<D>(f, 0, p1);
+#pragma unroll
+  for (int j = 0; j < D; ++j) p2[j] = 0.0f;
+  if (recorder) record_pick<D>(f, 0, i1, p1);
+
+  int cv = INT_MIN;  // this wave's best (uniform)
+  int ci = 0;
+  float cc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) cc[j] = 0.0f;
+  bool cached = false;
+
+  // Forced prefix (multi-start): one pick per pass, no exchange.
+  uint32_t s = 1;
+  for (; s < f.m && s < f.n_starts; ++s) {
+    if (!cached || box_bound<D>(p1, lo, hi) < __int_as_float(cv)) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p1), dm[k]);
+      wave_best([&](int k) { return dm[k]; }, cv, ci, cc);
+      cached = true;
+    }
+    i1 = load_forced<D>(f, s, p1);
+    if (recorder) record_pick<D>(f, s, i1, p1);
+  }
+
+  int ga[NW], gl_[NW];  // the workgroup's A and A' (uniform), kept while unchanged
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    ga[k] = 0;
+    gl_[k] = 0;
+  }
+  bool have2 = false;
+  bool la_valid = false;    // gl_ is the lookahead of the current state
+  int mu_v = INT_MIN;       // M: best A of the certainly-unchanged others
+  uint32_t mu_i = 0;
+  uint32_t e = 0;
+  const int K = min((int)f.replicas, 64 / NP);  // replicas a pair allows
+
+  // Bounding boxes of all workgroups: one exchange (number 1) of 2*D words.
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      s_wbox[j][wave] = lo[j];
+      s_wbox[D + j][wave] = hi[j];
+    }
+  }
+  lds_barrier();
+  ++e;
+  if (wave == 0) {
+    float bw[2 * D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      bw[j] = (lane < W) ? s_wbox[j][lane] : INFINITY;
+      bw[D + j] = (lane < W) ? s_wbox[D + j][lane] : -INFINITY;
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      bw[j] = wave_min_f(bw[j]);
+      bw[D + j] = wave_max_f(bw[D + j]);
+    }
+    uint64_t *bufs = f.xchg + (size_t)(e & 1u) * f.replicas * kSlotWords * kPageWords;
+    if (lane < K * 2 * D) {
+      const int rep = lane / (2 * D);
+      const int k = lane - rep * (2 * D);
+      float word = bw[0];
+#pragma unroll
+      for (int q = 1; q < 2 * D; ++q) {
+        float wq = bw[q];
+        asm volatile("" : "+v"(wq));
+        word = (k == q) ? wq : word;
+      }
+      xchg_store(bufs + ((size_t)rep * kSlotWords + k) * kPageWords + g,
+                 ((uint64_t)(uint32_t)__float_as_int(word) << 32) | e);
+    }
+    const uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * kPageWords;
+    bool ok = true;
+    if ((uint32_t)lane < G) {
+      uint64_t w[2 * D];
+      const uint64_t deadline = now_ticks() + f.spin_ticks;
+      for (;;) {
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < 2 * D; ++k) w[k] = xchg_load(buf + (size_t)k * kPageWords + lane);
+#pragma unroll
+        for (int k = 0; k < 2 * D; ++k) all = all && ((uint32_t)w[k] == e);
+        if (all) break;
+        if (now_ticks() > deadline) {
+          ok = false;
+          break;
+        }
+        for (uint32_t z = 0; z < f.backoff; ++z) __builtin_amdgcn_s_sleep(1);
+      }
+#pragma unroll
+      for (int k = 0; k < 2 * D; ++k) s_gbox[k][lane] = __int_as_float((int)(w[k] >> 32));
+    }
+    const bool all_ok = __ballot((uint32_t)lane < G && !ok) == 0;
+    if (lane == 0) s_ok = all_ok ? 1 : 0;
+  }
+  lds_barrier();
+  if (s_ok == 0) {
+    if (tid == 0) atomicOr(f.status, kStatusTimeout);
+    return;
+  }
+
+  while (s < f.m) {
+    // Apply the pending picks.  A wave whose box is at least as far from a
+    // pivot as its largest min-distance cannot change under that pivot.
+    const bool need1 = !cached || box_bound<D>(p1, lo, hi) < __int_as_float(cv);
+    const bool need2 = have2 && box_bound<D>(p2, lo, hi) < __int_as_float(cv);
+    if (need1) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p1), dm[k]);
+    }
+    if (need2) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p2), dm[k]);
+    }
+    if (need1 || need2) {
+      wave_best([&](int k) { return dm[k]; }, cv, ci, cc);
+      cached = true;
+    }
+
+    ++e;
+    const int par = (int)(e & 1u);
+    if (lane == 0) {
+      s_top[par][0][wave] = cv;
+      s_top[par][1][wave] = ci;
+#pragma unroll
+      for (int j = 0; j < D; ++j) s_top[par][2 + j][wave] = __float_as_int(cc[j]);
+      s_top[par][NW][wave] = (need1 || need2) ? 1 : 0;
+    }
+    lds_barrier();
+    int tw[NW + 1];
+#pragma unroll
+    for (int k = 0; k <= NW; ++k) tw[k] = (lane < W) ? s_top[par][k][lane] : 0;
+    if (lane >= W) tw[0] = INT_MIN;
+    // The first round computes A everywhere: after a forced prefix a wave's
+    // candidate is cached, but the workgroup's A has never been formed.
+    const bool changed = e == 2u || __ballot(lane < W && tw[NW] != 0) != 0;
+    if (changed) {
+      int gv;
+      const int wl = wave_argmax_lane<TIE>(tw[0], (uint32_t)tw[1], gv);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) ga[k] = readlane_i(tw[k], wl);
+    }
+    const bool needL = (changed || !la_valid) && beats<TIE>(ga[0], (uint32_t)ga[1], mu_v, mu_i);
+    if (changed) la_valid = false;
+    if (needL) {
+      // This workgroup could hold the next pick: every wave's lookahead.
+      la_valid = true;
+      float pa[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) pa[j] = __int_as_float(ga[2 + j]);
+      int lv = cv, li = ci;
+      float lc[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) lc[j] = cc[j];
+      if (box_bound<D>(pa, lo, hi) < __int_as_float(cv))
+        wave_best([&](int k) { return min_keep(sqdist<D>(c[k], pa), dm[k]); }, lv, li, lc);
+      if (lane == 0) {
+        s_la[par][0][wave] = lv;
+        s_la[par][1][wave] = li;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s_la[par][2 + j][wave] = __float_as_int(lc[j]);
+      }
+      lds_barrier();
+      int lw[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) lw[k] = (lane < W) ? s_la[par][k][lane] : 0;
+      if (lane >= W) lw[0] = INT_MIN;
+      int lgv;
+      const int wl2 = wave_argmax_lane<TIE>(lw[0], (uint32_t)lw[1], lgv);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) gl_[k] = readlane_i(lw[k], wl2);
+    }
+
+    // Exchange: wave 0 publishes the pair to K replicas and polls replica
+    // g mod K, one workgroup per lane, all NP words at once.
+    if (wave == 0) {
+      uint64_t *bufs = f.xchg + (size_t)(e & 1u) * f.replicas * kSlotWords * kPageWords;
+      if (lane < K * NP) {
+        const int rep = lane / NP;
+        const int k = lane - rep * NP;
+        // Each word goes through an empty asm: a select chain over a local
+        // array indexed by the lane would otherwise become a dynamic index
+        // and move the array to scratch memory.
+        int word = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          int wa = ga[q], wb = gl_[q];
+          asm volatile("" : "+v"(wa), "+v"(wb));
+          word = (k == q) ? wa : word;
+          word = (k == NW + q) ? wb : word;
+        }
+        // Byte offset formed per round (a hoisted 64-bit address would be
+        // spilled and reloaded on this path).
+        uint32_t poff = (uint32_t)(((rep * kSlotWords + k) * kPageWords + (int)g) * sizeof(uint64_t));
+        asm volatile("" : "+v"(poff));
+        xchg_store(reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(bufs) + poff),
+                   ((uint64_t)(uint32_t)word << 32) | e);
+      }
+      const uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * kPageWords;
+      // The lane offset is re-laundered on every poll so the NP per-lane
+      // addresses are formed just before their loads instead of being
+      // hoisted out of the retry loop (20 registers held across it).
+      uint32_t loff = (uint32_t)lane * (uint32_t)sizeof(uint64_t);
+      int x[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) x[k] = 0;
+      bool ok = true;
+      const bool live = (uint32_t)lane < G;
+      if (live) {
+        uint64_t w[NP];
+        const uint64_t deadline = now_ticks() + f.spin_ticks;
+        for (;;) {
+          bool all = true;
+          asm volatile("" : "+v"(loff));
+#pragma unroll
+          for (int k = 0; k < NP; ++k)
+            w[k] = xchg_load(reinterpret_cast<const uint64_t *>(
+                reinterpret_cast<const char *>(buf + (size_t)k * kPageWords) + loff));
+#pragma unroll
+          for (int k = 0; k < NP; ++k) all = all && ((uint32_t)w[k] == e);
+          if (all) break;
+          if (now_ticks() > deadline) {
+            ok = false;
+            break;
+          }
+          for (uint32_t z = 0; z < f.backoff; ++z) __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) x[k] = (int)(w[k] >> 32);
+      }
+      const bool valid = live && ok;
+      // p1: the best A.
+      int v1;
+      const int l1 = wave_argmax_lane<TIE>(valid ? x[0] : INT_MIN, (uint32_t)x[1], v1);
+      int r1[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) r1[k] = readlane_i(x[k], l1);
+      float q1[D], qa[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        q1[j] = __int_as_float(r1[2 + j]);
+        qa[j] = __int_as_float(x[2 + j]);
+      }
+      // Each A after p1, in the update's own arithmetic.
+      const float post = min_keep(sqdist<D>(qa, q1), __int_as_float(x[0]));
+      const bool moved = valid && lane != l1 && __float_as_int(post) != x[0];
+      const bool mine = lane == l1;
+      const int tv = mine ? x[NW] : ((valid && !moved) ? x[0] : INT_MIN);
+      const uint32_t ti = (uint32_t)(mine ? x[NW + 1] : x[1]);
+      int v2;
+      const int l2 = wave_argmax_lane<TIE>(tv, ti, v2);
+      const uint32_t i2 = (uint32_t)readlane_i((int)ti, l2);
+      const bool beaten = moved && !beats<TIE>(v2, i2, x[0], (uint32_t)x[1]);
+      const bool two = __ballot(beaten) == 0;
+      const bool all_ok = __ballot(live && !ok) == 0;
+      // Workgroups the round's picks certainly leave unchanged keep their A;
+      // M = the best of those A's other than this workgroup's.
+      float bl[D], bh[D], q2[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        bl[j] = s_gbox[j][lane];
+        bh[j] = s_gbox[D + j][lane];
+        q2[j] = __int_as_float(readlane_i(mine ? x[NW + 2 + j] : x[2 + j], l2));
+      }
+      const float avf = __int_as_float(x[0]);
+      const bool keep = valid && !(box_bound<D>(q1, bl, bh) < avf) &&
+                        !(two && box_bound<D>(q2, bl, bh) < avf);
+      int mv;
+      const int ml = wave_argmax_lane<TIE>((keep && (uint32_t)lane != g) ? x[0] : INT_MIN,
+                                           (uint32_t)x[1], mv);
+      const int mi = readlane_i(x[1], ml);
+      int r2[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const int cw = mine ? x[NW + 2 + j] : x[2 + j];
+        r2[j] = readlane_i(cw, l2);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) s_res[par][k] = r1[k];
+        s_res[par][NW] = v2;
+        s_res[par][NW + 1] = (int)i2;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s_res[par][NW + 2 + j] = r2[j];
+        s_res[par][NP] = two ? 1 : 0;
+        s_res[par][NP + 1] = all_ok ? 1 : 0;
+        s_res[par][NP + 2] = mv;
+        s_res[par][NP + 3] = mi;
+      }
+    }
+    lds_barrier();
+    int res[NP + 4];
+#pragma unroll
+    for (int k = 0; k < NP + 4; ++k) res[k] = s_res[par][k];
+    if (res[NP + 1] == 0) {
+      if (tid == 0) atomicOr(f.status, kStatusTimeout);
+      return;
+    }
+    // Uniform: keep the picks in scalar registers.
+    i1 = (uint32_t)__builtin_amdgcn_readfirstlane(res[1]);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      p1[j] = __int_as_float(__builtin_amdgcn_readfirstlane(res[2 + j]));
+      p2[j] = __int_as_float(__builtin_amdgcn_readfirstlane(res[NW + 2 + j]));
+    }
+    const uint32_t i2 = (uint32_t)__builtin_amdgcn_readfirstlane(res[NW + 1]);
+    mu_v = __builtin_amdgcn_readfirstlane(res[NP + 2]);
+    mu_i = (uint32_t)__builtin_amdgcn_readfirstlane(res[NP + 3]);
+    have2 = res[NP] != 0 && s + 1 < f.m;
+    if (recorder) {
+      record_pick<D>(f, s, i1, p1);
+      if (have2) record_pick<D>(f, s + 1, i2, p2);
+    }
+    s += have2 ? 2u : 1u;
   }
 }
 
@@ -418,6 +860,20 @@ static const KernelEntry kPruned[] = {
 const KernelEntry *pruned_table(size_t *count) {
   *count = sizeof(kPruned) / sizeof(kPruned[0]);
   return kPruned;
+}
+
+#define FPSX_PAIR(D, B, P)                                                          \
+  {D, B, P, kTieHighest, (const void *)&fps_pruned_pair<D, B, P, kTieHighest>}, \
+  {D, B, P, kTieLowest, (const void *)&fps_pruned_pair<D, B, P, kTieLowest>}
+
+static const KernelEntry kPair[] = {
+    FPSX_PAIR(3, 1024, 16), FPSX_PAIR(3, 1024, 8), FPSX_PAIR(3, 1024, 4),
+};
+#undef FPSX_PAIR
+
+const KernelEntry *pair_table(size_t *count) {
+  *count = sizeof(kPair) / sizeof(kPair[0]);
+  return kPair;
 }
 
 }  // namespace fpsx

@@ -17,6 +17,9 @@ struct PrunedParams {
 };
 
 const KernelEntry *pruned_table(size_t *count);
+// Paired form (same parameters): up to two picks per exchange; 2 <= G <= 64.
+const KernelEntry *pair_table(size_t *count);
+constexpr uint32_t kPairMaxGrid = 64;
 
 // Streaming pruned kernel: blocks of kSBlock positions in spatial order,
 // kSMetaWords LDS words per block (box, cached candidate, list entry).

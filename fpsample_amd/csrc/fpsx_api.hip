@@ -145,6 +145,14 @@ static const KernelEntry *find_pruned(int dim, int B, int ppt, int tie) {
   return nullptr;
 }
 
+static const KernelEntry *find_pair(int dim, int B, int ppt, int tie) {
+  size_t cnt;
+  const KernelEntry *t = pair_table(&cnt);
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].dim == dim && t[i].block == B && t[i].ppt == ppt && t[i].tie == tie) return &t[i];
+  return nullptr;
+}
+
 static const KernelEntry *find_stream(int dim, int B, int tie) {
   size_t cnt;
   const KernelEntry *t = stream_table(&cnt);
@@ -246,6 +254,15 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
       int got = std::sscanf(s.c_str(), "spruned:%d", &g);
       return plan_stream_pruned(c, n, tie, got == 1 ? (size_t)g : 0, p);
     }
+    if (std::sscanf(s.c_str(), "pair:%d:%d", &a, &b) == 2) {
+      const KernelEntry *k = find_pair((int)dim, a, b, tie);
+      if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no paired kernel for dim " +
+                                           std::to_string(dim) + " shape " + s);
+      if (plan_resident_shape(c, n, k, p, PlanKind::Pair) != FPSX_OK || p.grid < 2 ||
+          p.grid > kPairMaxGrid)
+        return fail(FPSX_ETOOBIG, "FPSX_PLAN: the paired kernel needs 2..64 workgroups: " + s);
+      return FPSX_OK;
+    }
     if (std::sscanf(s.c_str(), "pruned:%d:%d", &a, &b) == 2) {
       const KernelEntry *k = find_pruned((int)dim, a, b, tie);
       if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no pruned kernel for dim " +
@@ -278,7 +295,18 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
   // most passes touch few waves; the shape holding the most points per
   // workgroup keeps the exchange small.
   const char *np = std::getenv("FPSX_NO_PRUNE");
+  const char *npair = std::getenv("FPSX_NO_PAIR");
   if (dim == 3 && n >= 65536 && m >= 1024 && !(np && np[0] == '1')) {
+    // Paired form first (half the exchanges) where its grid limit allows.
+    if (!(npair && npair[0] == '1')) {
+      static const int kPa[][2] = {{1024, 16}, {1024, 8}, {1024, 4}};
+      for (auto &sh : kPa) {
+        const KernelEntry *k = find_pair(3, sh[0], sh[1], tie);
+        if (k && plan_resident_shape(c, n, k, p, PlanKind::Pair) == FPSX_OK && p.grid >= 2 &&
+            p.grid <= kPairMaxGrid)
+          return FPSX_OK;
+      }
+    }
     static const int kPr[][2] = {{1024, 16}, {256, 32}, {1024, 8}, {1024, 4}};
     for (auto &sh : kPr) {
       const KernelEntry *k = find_pruned(3, sh[0], sh[1], tie);
@@ -530,11 +558,12 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
   }
   PrunedParams pa;
   void *arg = &a;
-  if (p.kind == PlanKind::Pruned || p.kind == PlanKind::StreamPruned) {
+  if (p.kind == PlanKind::Pruned || p.kind == PlanKind::Pair || p.kind == PlanKind::StreamPruned) {
     HIP_TRY(c.sorted.reserve(n * 3 * sizeof(float)));
     HIP_TRY(c.perm.reserve(n * sizeof(uint32_t)));
     if (!c.spatial) c.spatial = new_spatial_scratch();
-    const uint32_t block = (p.kind == PlanKind::Pruned) ? (uint32_t)(64 * p.k->ppt) : (uint32_t)kSBlock;
+    const uint32_t block =
+        (p.kind == PlanKind::StreamPruned) ? (uint32_t)kSBlock : (uint32_t)(64 * p.k->ppt);
     int rc = spatial_order(d_pts, sp, ld, (uint32_t)n, block, (float *)c.sorted.p,
                            (uint32_t *)c.perm.p, *c.spatial, st);
     if (rc) return rc;

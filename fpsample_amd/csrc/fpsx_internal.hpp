@@ -49,7 +49,7 @@ struct Ctx {
   size_t events_used = 0;
 };
 
-enum class PlanKind : int { Resident = 0, Stream = 1, Pruned = 2, Npdu = 3, StreamPruned = 4 };
+enum class PlanKind : int { Resident = 0, Stream = 1, Pruned = 2, Npdu = 3, StreamPruned = 4, Pair = 5 };
 
 struct Plan {
   PlanKind kind = PlanKind::Resident;

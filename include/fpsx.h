@@ -61,11 +61,14 @@ int fpsx_device_status(void *stream);
  * The launch the planner picks for a cloud of n_points x dim and n_samples
  * picks under `tie`: kind 0 = register-resident, 1 = HBM-streaming,
  * 2 = register-resident with spatial order and per-wave pruning,
- * 4 = HBM-streaming with spatial order and per-block pruning; threads per
- * workgroup, points per thread (0 when streaming), workgroups, points per
- * workgroup.  FPSX_PLAN in the environment forces a plan: "resident:B:PPT",
- * "pruned:B:PPT", "stream:B[:G]", "generic:B[:G]", "spruned[:G]";
- * FPSX_NO_PRUNE=1 keeps the planner off the pruned kinds.
+ * 4 = HBM-streaming with spatial order and per-block pruning,
+ * 5 = kind 2 settling up to two picks per cross-workgroup exchange (2..64
+ * workgroups); threads per workgroup, points per thread (0 when streaming),
+ * workgroups, points per workgroup.  FPSX_PLAN in the environment forces a
+ * plan: "resident:B:PPT", "pruned:B:PPT", "pair:B:PPT", "stream:B[:G]",
+ * "generic:B[:G]", "spruned[:G]";
+ * FPSX_NO_PRUNE=1 keeps the planner off the pruned kinds, FPSX_NO_PAIR=1 off
+ * kind 5.
  */
 int fpsx_plan_info(size_t n_points, size_t dim, int tie, size_t n_samples, int *kind, int *block,
                    int *ppt, int *grid, int *chunk);

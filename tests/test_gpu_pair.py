@@ -1,0 +1,137 @@
+"""GPU parity of the paired pruned kernel (up to two picks per exchange).
+
+The paired kernel must give exactly the one-pick loop's picks: every shape is
+forced through FPSX_PLAN and compared with the oracle on clustered, uniform,
+duplicate-heavy (exact ties, repeated picks once every distance is zero) and
+special-value clouds, with single and multi-start prefixes, both tie rules
+(the bucket entry uses lowest-position ties), odd and even sample counts, and
+at the benchmark size against the one-pick pruned kernel.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.capi import COLS
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture
+def plan_env(monkeypatch):
+    def set_plan(p):
+        if p is None:
+            monkeypatch.delenv("FPSX_PLAN", raising=False)
+        else:
+            monkeypatch.setenv("FPSX_PLAN", p)
+    yield set_plan
+    monkeypatch.delenv("FPSX_PLAN", raising=False)
+
+
+def run_fps(gpu, pc, m, starts, layout=COLS):
+    rc, out = gpu.fps(pc, m, starts, layout=layout)
+    assert rc == 0, f"rc={rc}: {gpu.last_error()}"
+    return out
+
+
+def gauss_cloud(n, seed):
+    rng = np.random.default_rng(seed)
+    centers = rng.random((64, 3))
+    lab = rng.integers(0, 64, n)
+    return (centers[lab] + 0.02 * rng.standard_normal((n, 3))).astype(np.float32)
+
+
+PAIR_SHAPES = ["pair:1024:16", "pair:1024:8", "pair:1024:4"]
+
+
+@pytest.mark.parametrize("shape", PAIR_SHAPES)
+def test_pair_shapes(gpu, plan_env, shape):
+    plan_env(shape)
+    rng = np.random.default_rng(41)
+    clouds = [("gauss", gauss_cloud(40000, 3)),
+              ("lattice_dups", np.repeat(rng.integers(0, 6, (6000, 3)), 3, axis=0).astype(np.float32)),
+              ("uniform", rng.random((30000, 3)).astype(np.float32))]
+    for name, pc in clouds:
+        info = gpu.plan_info(len(pc), 3, m=2001)
+        assert info["kind"] == 5 and 2 <= info["grid"] <= 64, info
+        for m, starts in ((2001, 0), (2000, [len(pc) - 1, 7, 11])):
+            log(f"{shape} {name} m={m} starts={starts} plan={info}")
+            want = O.fps(pc, m, starts)
+            np.testing.assert_array_equal(run_fps(gpu, pc, m, starts), want,
+                                          err_msg=f"{shape} {name} m={m}")
+
+
+def test_pair_long_forced_prefix(gpu, plan_env):
+    # A long forced prefix (multi-start / resume) leaves every wave's
+    # candidate cached before the first exchange; workgroups the last forced
+    # pick does not touch must still publish their best.
+    plan_env("pair:1024:4")
+    pc = gauss_cloud(40000, 6)
+    want = O.fps(pc, 1500, 0)
+    for k in (2, 100, 900):
+        log(f"forced prefix of {k}")
+        np.testing.assert_array_equal(run_fps(gpu, pc, 1500, want[:k]), want)
+
+
+def test_pair_all_distances_zero(gpu, plan_env):
+    # 20 distinct sites, 40000 points: after 20 picks every min-distance is
+    # zero and the loop repeats the highest index; pairs must do the same.
+    plan_env("pair:1024:4")
+    rng = np.random.default_rng(42)
+    sites = rng.random((20, 3)).astype(np.float32)
+    pc = sites[rng.integers(0, 20, 40000)]
+    for m in (64, 65):
+        log(f"zero distances m={m}")
+        np.testing.assert_array_equal(run_fps(gpu, pc, m, 5), O.fps(pc, m, 5))
+
+
+def test_pair_special_values(gpu, plan_env):
+    plan_env("pair:1024:4")
+    rng = np.random.default_rng(43)
+    pc = (rng.random((9000, 3)) * 4e19).astype(np.float32)
+    pc[[5, 700, 4000]] = np.nan
+    pc[9] = np.inf
+    pc[10, 1] = -np.inf
+    log("special values: nan/inf/huge")
+    np.testing.assert_array_equal(run_fps(gpu, pc, 600, 0), O.fps(pc, 600, 0))
+    tiny = (rng.random((9000, 3)) * 1e-21).astype(np.float32)
+    log("special values: denormal distances")
+    np.testing.assert_array_equal(run_fps(gpu, tiny, 600, 0), O.fps(tiny, 600, 0))
+
+
+def test_pair_lowest_ties_bucket_entry(gpu, plan_env):
+    # Bucket entry: lowest-position ties over the KD-line order.
+    plan_env("pair:1024:4")
+    rng = np.random.default_rng(44)
+    lat = rng.integers(0, 40, (100000, 3)).astype(np.float32)
+    log("bucket entry, lattice, pair:1024:4")
+    rc, out = gpu.kdline(lat, 1500, 123, 7)
+    assert rc == 0, gpu.last_error()
+    np.testing.assert_array_equal(out, O.kdline_quickfps(lat, 1500, 7, 123))
+
+
+def test_pair_full_size_matches_one_pick_kernel(gpu, plan_env):
+    # The benchmark cloud (2^20 Gaussian-cluster points, 16384 picks): the
+    # paired kernel and the one-pick pruned kernel agree on every pick, and
+    # the first 512 match the oracle.
+    pc = gauss_cloud(1 << 20, 1)
+    plan_env("pruned:1024:16")
+    one = run_fps(gpu, pc, 16384, 0)
+    plan_env("pair:1024:16")
+    log(f"full size plan={gpu.plan_info(len(pc), 3, m=16384)}")
+    two = run_fps(gpu, pc, 16384, 0)
+    np.testing.assert_array_equal(two, one)
+    np.testing.assert_array_equal(two[:512], O.fps(pc, 512, 0))
+
+
+def test_pair_rejects_grids_outside_range(gpu, plan_env):
+    plan_env("pair:1024:16")
+    pc = np.random.default_rng(45).random((9000, 3)).astype(np.float32)
+    rc, _ = gpu.fps(pc, 100, 0)
+    assert rc != 0 and "2..64" in gpu.last_error()

@@ -264,7 +264,7 @@ def test_pruned_auto_plan_and_bucket_order(gpu):
     pc = c2_cloud(120000, seed=5)
     info = gpu.plan_info(len(pc), 3, m=2048)
     log(f"auto plan {info}")
-    assert info["kind"] == 2
+    assert info["kind"] == 5  # paired pruned kernel (8 workgroups)
     np.testing.assert_array_equal(run_fps(gpu, pc, 2048, 77), O.fps(pc, 2048, 77))
     # bucket entry: lowest-position ties over the KD-line order, pruned plan
     lat = rng.integers(0, 40, (100000, 3)).astype(np.float32)
