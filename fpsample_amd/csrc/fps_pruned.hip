@@ -240,7 +240,8 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   constexpr int PER_WG = B * PPT;
   __shared__ int s_top[2][NW + 1][W];  // wave bests + "changed" flags, by exchange parity
   __shared__ int s_la[2][NW][W];       // wave lookahead bests
-  __shared__ int s_res[2][NP + 4];     // p1, p2, two-pick flag, all-arrived flag, M
+  __shared__ int s_res[2][NP + 2];     // p1, p2, two-pick flag, all-arrived flag
+  __shared__ int s_mu[2][2];           // M of the round of this parity
   __shared__ float s_gbox[2 * D][64];  // every workgroup's bounding box
   __shared__ float s_wbox[2 * D][W];   // this workgroup's wave boxes
   __shared__ int s_ok;
@@ -356,9 +357,16 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   int mu_v = INT_MIN;       // M: best A of the certainly-unchanged others
   uint32_t mu_i = 0;
   uint32_t e = 0;
-  const int K = min((int)f.replicas, 64 / NP);  // replicas a pair allows
+  // Replicas: at most 4 for pairs (measured on gauss1m: 4 replicas 31.7 ms
+  // per launch, 6 replicas 32.4 ms; fewer polls per page do not pay for the
+  // extra publishes).
+  const int K = min(min((int)f.replicas, 64 / NP), 4);
 
   // Bounding boxes of all workgroups: one exchange (number 1) of 2*D words.
+  if (tid < 2) {
+    s_mu[tid][0] = INT_MIN;
+    s_mu[tid][1] = 0;
+  }
   if (lane == 0) {
 #pragma unroll
     for (int j = 0; j < D; ++j) {
@@ -456,6 +464,9 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
 #pragma unroll
     for (int k = 0; k <= NW; ++k) tw[k] = (lane < W) ? s_top[par][k][lane] : 0;
     if (lane >= W) tw[0] = INT_MIN;
+    // M of the previous round (written by wave 0 before this barrier).
+    mu_v = s_mu[par ^ 1][0];
+    mu_i = (uint32_t)s_mu[par ^ 1][1];
     // The first round computes A everywhere: after a forced prefix a wave's
     // candidate is cached, but the workgroup's A has never been formed.
     const bool changed = e == 2u || __ballot(lane < W && tw[NW] != 0) != 0;
@@ -496,6 +507,7 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
       for (int k = 0; k < NW; ++k) gl_[k] = readlane_i(lw[k], wl2);
     }
 
+    int sv = INT_MIN, si = 0;  // wave 0: this lane's workgroup's A (value, index)
     // Exchange: wave 0 publishes the pair to K replicas and polls replica
     // g mod K, one workgroup per lane, all NP words at once.
     if (wave == 0) {
@@ -578,22 +590,8 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
       const bool beaten = moved && !beats<TIE>(v2, i2, x[0], (uint32_t)x[1]);
       const bool two = __ballot(beaten) == 0;
       const bool all_ok = __ballot(live && !ok) == 0;
-      // Workgroups the round's picks certainly leave unchanged keep their A;
-      // M = the best of those A's other than this workgroup's.
-      float bl[D], bh[D], q2[D];
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        bl[j] = s_gbox[j][lane];
-        bh[j] = s_gbox[D + j][lane];
-        q2[j] = __int_as_float(readlane_i(mine ? x[NW + 2 + j] : x[2 + j], l2));
-      }
-      const float avf = __int_as_float(x[0]);
-      const bool keep = valid && !(box_bound<D>(q1, bl, bh) < avf) &&
-                        !(two && box_bound<D>(q2, bl, bh) < avf);
-      int mv;
-      const int ml = wave_argmax_lane<TIE>((keep && (uint32_t)lane != g) ? x[0] : INT_MIN,
-                                           (uint32_t)x[1], mv);
-      const int mi = readlane_i(x[1], ml);
+      sv = valid ? x[0] : INT_MIN;
+      si = x[1];
       int r2[D];
 #pragma unroll
       for (int j = 0; j < D; ++j) {
@@ -609,14 +607,12 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
         for (int j = 0; j < D; ++j) s_res[par][NW + 2 + j] = r2[j];
         s_res[par][NP] = two ? 1 : 0;
         s_res[par][NP + 1] = all_ok ? 1 : 0;
-        s_res[par][NP + 2] = mv;
-        s_res[par][NP + 3] = mi;
       }
     }
     lds_barrier();
-    int res[NP + 4];
+    int res[NP + 2];
 #pragma unroll
-    for (int k = 0; k < NP + 4; ++k) res[k] = s_res[par][k];
+    for (int k = 0; k < NP + 2; ++k) res[k] = s_res[par][k];
     if (res[NP + 1] == 0) {
       if (tid == 0) atomicOr(f.status, kStatusTimeout);
       return;
@@ -629,9 +625,31 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
       p2[j] = __int_as_float(__builtin_amdgcn_readfirstlane(res[NW + 2 + j]));
     }
     const uint32_t i2 = (uint32_t)__builtin_amdgcn_readfirstlane(res[NW + 1]);
-    mu_v = __builtin_amdgcn_readfirstlane(res[NP + 2]);
-    mu_i = (uint32_t)__builtin_amdgcn_readfirstlane(res[NP + 3]);
-    have2 = res[NP] != 0 && s + 1 < f.m;
+    const bool two = res[NP] != 0;
+    have2 = two && s + 1 < f.m;
+    if (wave == 0) {
+      // Workgroups the round's picks certainly leave unchanged keep their A;
+      // M = the best of those A's other than this workgroup's.  Formed after
+      // the hand-off, while the other waves start the next round, and read
+      // by every wave after the next round's first barrier.
+      float bl[D], bh[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        bl[j] = s_gbox[j][lane];
+        bh[j] = s_gbox[D + j][lane];
+      }
+      const float avf = __int_as_float(sv);
+      const bool keep = sv != INT_MIN && !(box_bound<D>(p1, bl, bh) < avf) &&
+                        !(two && box_bound<D>(p2, bl, bh) < avf);
+      int mv;
+      const int ml = wave_argmax_lane<TIE>((keep && (uint32_t)lane != g) ? sv : INT_MIN,
+                                           (uint32_t)si, mv);
+      const int mi = readlane_i(si, ml);
+      if (lane == 0) {
+        s_mu[par][0] = mv;
+        s_mu[par][1] = mi;
+      }
+    }
     if (recorder) {
       record_pick<D>(f, s, i1, p1);
       if (have2) record_pick<D>(f, s + 1, i2, p2);
