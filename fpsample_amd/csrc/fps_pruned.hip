@@ -51,15 +51,17 @@ __device__ __forceinline__ float wave_max_f(float v) {
 
 // Lower bound of the squared distance from p to the box [lo, hi], in the
 // point distance's operand order (src/_ext/KDNode.h:105-118 does the same).
+// The gap along j is max(p - hi, lo - p, 0), written without branches: it is
+// p - hi when p > hi, lo - p when p < lo and 0 otherwise, as in the branchy
+// form, including NaN p (both differences NaN, max returns 0) and an empty
+// box (lo = +inf, hi = -inf: gap +inf).
 template <int D>
 __device__ __forceinline__ float box_bound(const float (&p)[D], const float (&lo)[D],
                                            const float (&hi)[D]) {
   float acc = 0.0f;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    float g = 0.0f;
-    if (p[j] > hi[j]) g = p[j] - hi[j];
-    else if (p[j] < lo[j]) g = lo[j] - p[j];
+    const float g = fmaxf(fmaxf(p[j] - hi[j], lo[j] - p[j]), 0.0f);
     acc = (j == 0) ? g * g : acc + g * g;
   }
   return acc;
@@ -479,16 +481,31 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
     const bool needL = (changed || !la_valid) && beats<TIE>(ga[0], (uint32_t)ga[1], mu_v, mu_i);
     if (changed) la_valid = false;
     if (needL) {
-      // This workgroup could hold the next pick: every wave's lookahead.
+      // This workgroup could hold the next pick: its lookahead.  A wave
+      // whose best point keeps its value under A (the update's own
+      // arithmetic) still has that best after the hypothetical update; L =
+      // the best of those.  Only a wave whose best moves and could still beat
+      // L sweeps its points -- normally just the wave holding A.  Every wave
+      // derives the same set from the wave bests in LDS.
       la_valid = true;
-      float pa[D];
+      float pa[D], tc[D];
 #pragma unroll
-      for (int j = 0; j < D; ++j) pa[j] = __int_as_float(ga[2 + j]);
+      for (int j = 0; j < D; ++j) {
+        pa[j] = __int_as_float(ga[2 + j]);
+        tc[j] = __int_as_float(tw[2 + j]);
+      }
+      const float tpost = min_keep(sqdist<D>(tc, pa), __int_as_float(tw[0]));
+      const bool texact = lane < W && __float_as_int(tpost) == tw[0];
+      int lbv;
+      const int lbl = wave_argmax_lane<TIE>(texact ? tw[0] : INT_MIN, (uint32_t)tw[1], lbv);
+      const uint32_t lbi = (uint32_t)readlane_i(tw[1], lbl);
+      const uint64_t sweepers =
+          __ballot(lane < W && !texact && beats<TIE>(tw[0], (uint32_t)tw[1], lbv, lbi));
       int lv = cv, li = ci;
       float lc[D];
 #pragma unroll
       for (int j = 0; j < D; ++j) lc[j] = cc[j];
-      if (box_bound<D>(pa, lo, hi) < __int_as_float(cv))
+      if ((sweepers >> wave) & 1ull)
         wave_best([&](int k) { return min_keep(sqdist<D>(c[k], pa), dm[k]); }, lv, li, lc);
       if (lane == 0) {
         s_la[par][0][wave] = lv;
