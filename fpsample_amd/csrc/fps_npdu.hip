@@ -205,6 +205,391 @@ __global__ void __launch_bounds__(B) fps_npdu(NpduParams a) {
 }
 
 // ---------------------------------------------------------------------------
+// Paired form (3-D, 2 <= G <= 64): one cross-workgroup exchange settles up to
+// two picks, by the rule of the paired pruned kernel (fps_pruned.hip) with the
+// windowed update in place of the full one:
+//   every workgroup publishes its best A and its lookahead A' = its best after
+//   a hypothetical window update around A (only the waves under that window
+//   can change, and only a wave whose best point moves sweeps);
+//   p1 = the best A (workgroup w1); w1's best after p1 is A' of w1; another
+//   workgroup keeps A unless A lies in p1's window and its value drops there,
+//   else it is known only as "below A"; p2 = the best known best when it beats
+//   every such bound, else the round yields p1 alone;
+//   a workgroup forms A' only when it could be w1: its A beats M, the best A
+//   of the other workgroups whose index range misses the last round's
+//   windows (those certainly kept their A; the test is exact here).
+// Ties go to the lowest index (strict `>`), everywhere.
+// ---------------------------------------------------------------------------
+template <int B, int PPT>
+__global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
+  constexpr int D = 3;
+  constexpr int TIE = kTieLowest;
+  constexpr int W = B / 64;
+  constexpr int NW = 2 + D;   // value, index, coordinates
+  constexpr int NP = 2 * NW;  // words published per workgroup: A, then A'
+  __shared__ int s_top[2][NW + 1][W];  // wave bests + "changed" flags, by exchange parity
+  __shared__ int s_la[2][NW][W];       // wave lookahead bests
+  __shared__ int s_res[2][NP + 2];     // p1, p2, two-pick flag, all-arrived flag
+  __shared__ int s_mu[2][2];           // M of the round of this parity
+
+  const FpsParams &f = a.fps;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const uint32_t G = gridDim.x;
+  const uint32_t g = blockIdx.x;
+  const uint32_t n = f.n;
+  const uint32_t wbase =
+      __builtin_amdgcn_readfirstlane(g * (uint32_t)(B * PPT) + (uint32_t)(wave * 64 * PPT));
+  const uint32_t wlast = wbase + (uint32_t)(64 * PPT - 1);
+  const bool recorder = (g == 0 && tid >= B - 64);
+
+  // Unconditional loads of a clamped index; padding slots are then overwritten.
+  float c[PPT][D];
+  float dm[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint32_t i = wbase + (uint32_t)(k * 64 + lane);
+    const bool live = i < n;
+    const uint32_t ic = live ? i : n - 1u;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const float v = f.pts[(size_t)ic * f.sp + (size_t)j * f.ld];
+      c[k][j] = live ? v : 0.0f;
+    }
+    dm[k] = live ? f.init : -1.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) settle<D>(c[k]);
+
+  uint64_t st = f.starts[0];
+  if (st >= n) {
+    if (tid == 0) atomicOr(f.status, kStatusBadStart);
+    st = 0;
+  }
+  uint32_t i1 = (uint32_t)st;
+  float p1[D], p2[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    p1[j] = f.pts[(size_t)i1 * f.sp + (size_t)j * f.ld];
+    p2[j] = 0.0f;
+  }
+  settle<D>(p1);
+  if (recorder) record_pick<D>(f, 0, i1, p1);
+
+  // Window of a pick: [idx - w/2, idx + w/2] shifted into [0, n-1]
+  // (src/lib.cpp:296-309).
+  const int64_t hw = (int64_t)(a.window / 2);
+  auto window = [&](uint32_t idx, uint32_t &wl, uint32_t &wh) {
+    int64_t l = (int64_t)idx - hw;
+    int64_t h = (int64_t)idx + hw;
+    if (l < 0) {
+      h -= l;
+      l = 0;
+    }
+    if (h >= (int64_t)n) {
+      l = l - (h - (int64_t)n + 1);
+      if (l < 0) l = 0;
+      h = (int64_t)n - 1;
+    }
+    wl = (uint32_t)l;
+    wh = (uint32_t)h;
+  };
+  // The window's update of this wave's slots; false when it does not reach the wave.
+  auto apply = [&](const float(&p)[D], uint32_t wl, uint32_t wh) -> bool {
+    if (!(wbase <= wh && wlast >= wl)) return false;
+    uint32_t ib = wbase + (uint32_t)lane;
+    asm volatile("" : "+v"(ib));
+    const int k_lo = (int)((wl > wbase ? wl - wbase : 0u) >> 6);
+    const int k_hi = (int)((wh - wbase) >> 6);
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      if (k < k_lo || k > k_hi) continue;
+      const uint32_t i = ib + (uint32_t)(k * 64);
+      const float d = sqdist<D>(c[k], p);
+      dm[k] = (i >= wl && i <= wh) ? min_keep(d, dm[k]) : dm[k];
+    }
+    return true;
+  };
+  // Best (value, index, coordinates) of this wave over val(k): the lowest
+  // slot, then the lowest lane, among equal values, i.e. the lowest index.
+  auto wave_best = [&](auto val, int &ov, int &oi, float(&oc)[D]) {
+    int bv = INT_MIN;
+    int bk = 0;
+    float bc[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) bc[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int b = __float_as_int(val(k));
+      const bool take = b > bv;
+      bv = take ? b : bv;
+      bk = take ? k : bk;
+#pragma unroll
+      for (int j = 0; j < D; ++j) bc[j] = take ? c[k][j] : bc[j];
+    }
+    const uint32_t bidx = wbase + (uint32_t)(bk * 64 + lane);
+    int wv;
+    const int wl = wave_argmax_lane<TIE>(bv, bidx, wv);
+#pragma unroll
+    for (int j = 0; j < D; ++j) oc[j] = readlane_f(bc[j], wl);
+    ov = wv;
+    oi = readlane_i((int)bidx, wl);
+  };
+
+  int cv = INT_MIN;  // this wave's best (uniform)
+  int ci = 0;
+  float cc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) cc[j] = 0.0f;
+  bool cached = false;
+
+  int ga[NW], gl_[NW];  // the workgroup's A and A' (uniform), kept while unchanged
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    ga[k] = 0;
+    gl_[k] = 0;
+  }
+  bool la_valid = false;
+  int mu_v = INT_MIN;
+  uint32_t mu_i = 0;
+  bool have2 = false;
+  uint32_t w1l = 0, w1h = n - 1u, w2l = 0, w2h = 0;  // the start updates every point
+  uint32_t e = 0;
+  const int K = min(min((int)f.replicas, 64 / NP), 4);
+  if (tid < 2) {
+    s_mu[tid][0] = INT_MIN;
+    s_mu[tid][1] = 0;
+  }
+  lds_barrier();
+
+  uint32_t s = 1;
+  while (s < f.m) {
+    bool upd = apply(p1, w1l, w1h);
+    if (have2) upd = apply(p2, w2l, w2h) || upd;
+    if (upd || !cached) {
+      wave_best([&](int k) { return dm[k]; }, cv, ci, cc);
+      cached = true;
+    }
+
+    ++e;
+    const int par = (int)(e & 1u);
+    if (lane == 0) {
+      s_top[par][0][wave] = cv;
+      s_top[par][1][wave] = ci;
+#pragma unroll
+      for (int j = 0; j < D; ++j) s_top[par][2 + j][wave] = __float_as_int(cc[j]);
+      s_top[par][NW][wave] = upd ? 1 : 0;
+    }
+    lds_barrier();
+    int tw[NW + 1];
+#pragma unroll
+    for (int k = 0; k <= NW; ++k) tw[k] = (lane < W) ? s_top[par][k][lane] : 0;
+    if (lane >= W) tw[0] = INT_MIN;
+    mu_v = s_mu[par ^ 1][0];
+    mu_i = (uint32_t)s_mu[par ^ 1][1];
+    const bool changed = e == 1u || __ballot(lane < W && tw[NW] != 0) != 0;
+    if (changed) {
+      int gv;
+      const int wl = wave_argmax_lane<TIE>(tw[0], (uint32_t)tw[1], gv);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) ga[k] = readlane_i(tw[k], wl);
+    }
+    const bool needL = (changed || !la_valid) && beats<TIE>(ga[0], (uint32_t)ga[1], mu_v, mu_i);
+    if (changed) la_valid = false;
+    if (needL) {
+      // Lookahead: the window around A.  A wave whose best point lies outside
+      // it, or keeps its value under it, still has that best; L = the best of
+      // those; only a wave whose best moves and could beat L sweeps.
+      la_valid = true;
+      float pa[D], tc[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        pa[j] = __int_as_float(ga[2 + j]);
+        tc[j] = __int_as_float(tw[2 + j]);
+      }
+      uint32_t al, ah;
+      window((uint32_t)ga[1], al, ah);
+      const uint32_t ti = (uint32_t)tw[1];
+      const bool inw = ti >= al && ti <= ah;
+      const float tpost = min_keep(sqdist<D>(tc, pa), __int_as_float(tw[0]));
+      const bool texact = lane < W && (!inw || __float_as_int(tpost) == tw[0]);
+      int lbv;
+      const int lbl = wave_argmax_lane<TIE>(texact ? tw[0] : INT_MIN, ti, lbv);
+      const uint32_t lbi = (uint32_t)readlane_i(tw[1], lbl);
+      const uint64_t sweepers = __ballot(lane < W && !texact && beats<TIE>(tw[0], ti, lbv, lbi));
+      int lv = cv, li = ci;
+      float lc[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) lc[j] = cc[j];
+      if ((sweepers >> wave) & 1ull) {
+        uint32_t ib = wbase + (uint32_t)lane;
+        asm volatile("" : "+v"(ib));
+        wave_best(
+            [&](int k) {
+              const uint32_t i = ib + (uint32_t)(k * 64);
+              return (i >= al && i <= ah) ? min_keep(sqdist<D>(c[k], pa), dm[k]) : dm[k];
+            },
+            lv, li, lc);
+      }
+      if (lane == 0) {
+        s_la[par][0][wave] = lv;
+        s_la[par][1][wave] = li;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s_la[par][2 + j][wave] = __float_as_int(lc[j]);
+      }
+      lds_barrier();
+      int lw[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) lw[k] = (lane < W) ? s_la[par][k][lane] : 0;
+      if (lane >= W) lw[0] = INT_MIN;
+      int lgv;
+      const int wl2 = wave_argmax_lane<TIE>(lw[0], (uint32_t)lw[1], lgv);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) gl_[k] = readlane_i(lw[k], wl2);
+    }
+
+    int sv = INT_MIN, si = 0;  // wave 0: this lane's workgroup's A (value, index)
+    if (wave == 0) {
+      uint64_t *bufs = f.xchg + (size_t)(e & 1u) * f.replicas * kSlotWords * kPageWords;
+      if (lane < K * NP) {
+        const int rep = lane / NP;
+        const int k = lane - rep * NP;
+        // Each word goes through an empty asm: a select chain over a local
+        // array indexed by the lane would otherwise become a dynamic index.
+        int word = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          int wa = ga[q], wb = gl_[q];
+          asm volatile("" : "+v"(wa), "+v"(wb));
+          word = (k == q) ? wa : word;
+          word = (k == NW + q) ? wb : word;
+        }
+        uint32_t poff = (uint32_t)(((rep * kSlotWords + k) * kPageWords + (int)g) * sizeof(uint64_t));
+        asm volatile("" : "+v"(poff));
+        xchg_store(reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(bufs) + poff),
+                   ((uint64_t)(uint32_t)word << 32) | e);
+      }
+      const uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * kPageWords;
+      uint32_t loff = (uint32_t)lane * (uint32_t)sizeof(uint64_t);
+      int x[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) x[k] = 0;
+      bool ok = true;
+      const bool live = (uint32_t)lane < G;
+      if (live) {
+        uint64_t w[NP];
+        const uint64_t deadline = now_ticks() + f.spin_ticks;
+        for (;;) {
+          bool all = true;
+          asm volatile("" : "+v"(loff));
+#pragma unroll
+          for (int k = 0; k < NP; ++k)
+            w[k] = xchg_load(reinterpret_cast<const uint64_t *>(
+                reinterpret_cast<const char *>(buf + (size_t)k * kPageWords) + loff));
+#pragma unroll
+          for (int k = 0; k < NP; ++k) all = all && ((uint32_t)w[k] == e);
+          if (all) break;
+          if (now_ticks() > deadline) {
+            ok = false;
+            break;
+          }
+          for (uint32_t z = 0; z < f.backoff; ++z) __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) x[k] = (int)(w[k] >> 32);
+      }
+      const bool valid = live && ok;
+      // p1: the best A.
+      int v1;
+      const int l1 = wave_argmax_lane<TIE>(valid ? x[0] : INT_MIN, (uint32_t)x[1], v1);
+      int r1[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) r1[k] = readlane_i(x[k], l1);
+      float q1[D], qa[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        q1[j] = __int_as_float(r1[2 + j]);
+        qa[j] = __int_as_float(x[2 + j]);
+      }
+      uint32_t ql, qh;
+      window((uint32_t)r1[1], ql, qh);
+      // Each A after p1's window update, in the update's own arithmetic.
+      const uint32_t xi = (uint32_t)x[1];
+      const float post = (xi >= ql && xi <= qh) ? min_keep(sqdist<D>(qa, q1), __int_as_float(x[0]))
+                                                : __int_as_float(x[0]);
+      const bool moved = valid && lane != l1 && __float_as_int(post) != x[0];
+      const bool mine = lane == l1;
+      const int tv = mine ? x[NW] : ((valid && !moved) ? x[0] : INT_MIN);
+      const uint32_t tix = (uint32_t)(mine ? x[NW + 1] : x[1]);
+      int v2;
+      const int l2 = wave_argmax_lane<TIE>(tv, tix, v2);
+      const uint32_t i2 = (uint32_t)readlane_i((int)tix, l2);
+      const bool beaten = moved && !beats<TIE>(v2, i2, x[0], xi);
+      const bool two = __ballot(beaten) == 0;
+      const bool all_ok = __ballot(live && !ok) == 0;
+      sv = valid ? x[0] : INT_MIN;
+      si = x[1];
+      int r2[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const int cw = mine ? x[NW + 2 + j] : x[2 + j];
+        r2[j] = readlane_i(cw, l2);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) s_res[par][k] = r1[k];
+        s_res[par][NW] = v2;
+        s_res[par][NW + 1] = (int)i2;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s_res[par][NW + 2 + j] = r2[j];
+        s_res[par][NP] = two ? 1 : 0;
+        s_res[par][NP + 1] = all_ok ? 1 : 0;
+      }
+    }
+    lds_barrier();
+    int res[NP + 2];
+#pragma unroll
+    for (int k = 0; k < NP + 2; ++k) res[k] = s_res[par][k];
+    if (res[NP + 1] == 0) {
+      if (tid == 0) atomicOr(f.status, kStatusTimeout);
+      return;
+    }
+    i1 = (uint32_t)__builtin_amdgcn_readfirstlane(res[1]);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      p1[j] = __int_as_float(__builtin_amdgcn_readfirstlane(res[2 + j]));
+      p2[j] = __int_as_float(__builtin_amdgcn_readfirstlane(res[NW + 2 + j]));
+    }
+    const uint32_t i2 = (uint32_t)__builtin_amdgcn_readfirstlane(res[NW + 1]);
+    const bool two = res[NP] != 0;
+    have2 = two && s + 1 < f.m;
+    window(i1, w1l, w1h);
+    if (two) window(i2, w2l, w2h);
+    if (wave == 0) {
+      // M: the best A of the other workgroups whose index range misses the
+      // windows of this round's picks (formed after the hand-off).
+      const uint32_t qlo = (uint32_t)lane * (uint32_t)(B * PPT);
+      const uint32_t qhi = qlo + (uint32_t)(B * PPT - 1);
+      const bool keep = sv != INT_MIN && (uint32_t)lane != g && !(qlo <= w1h && qhi >= w1l) &&
+                        !(two && qlo <= w2h && qhi >= w2l);
+      int mv;
+      const int ml = wave_argmax_lane<TIE>(keep ? sv : INT_MIN, (uint32_t)si, mv);
+      const int mi = readlane_i(si, ml);
+      if (lane == 0) {
+        s_mu[par][0] = mv;
+        s_mu[par][1] = mi;
+      }
+    }
+    if (recorder) {
+      record_pick<D>(f, s, i1, p1);
+      if (have2) record_pick<D>(f, s + 1, i2, p2);
+    }
+    s += have2 ? 2u : 1u;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Streaming form for clouds beyond the register file, any dimension.  The
 // min-distances live in HBM (f.dmin, by index) and the cloud is cut into
 // blocks of kNpduBlock consecutive indices; block b belongs to workgroup
@@ -370,6 +755,19 @@ static const KernelEntry kNpdu[] = {
 const KernelEntry *npdu_table(size_t *count) {
   *count = sizeof(kNpdu) / sizeof(kNpdu[0]);
   return kNpdu;
+}
+
+#define FPSX_NPDU_PAIR(B, P) {3, B, P, kTieLowest, (const void *)&fps_npdu_pair<B, P>}
+
+static const KernelEntry kNpduPair[] = {
+    FPSX_NPDU_PAIR(1024, 2), FPSX_NPDU_PAIR(1024, 4), FPSX_NPDU_PAIR(1024, 8),
+    FPSX_NPDU_PAIR(1024, 16),
+};
+#undef FPSX_NPDU_PAIR
+
+const KernelEntry *npdu_pair_table(size_t *count) {
+  *count = sizeof(kNpduPair) / sizeof(kNpduPair[0]);
+  return kNpduPair;
 }
 
 }  // namespace fpsx

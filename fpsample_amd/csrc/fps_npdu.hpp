@@ -16,6 +16,9 @@ struct NpduParams {
 // Entries: dim 3 (coordinates in registers) and dim 0 (any dimension,
 // coordinates read from memory for the window only); tie is always lowest.
 const KernelEntry *npdu_table(size_t *count);
+// Paired form (dim 3): up to two picks per exchange; 2 <= G <= 64.
+const KernelEntry *npdu_pair_table(size_t *count);
+constexpr uint32_t kNpduPairMaxGrid = 64;
 
 // Streaming form (any dimension, ppt 0): blocks of kNpduBlock indices; LDS
 // holds dim floats of pivot plus 2 words per block of the workgroup.

@@ -388,12 +388,31 @@ static int plan_npdu_stream(Ctx &c, size_t n, size_t dim, size_t want_g, Plan &p
   return FPSX_OK;
 }
 
+static const KernelEntry *find_npdu_pair(int B, int ppt) {
+  size_t cnt;
+  const KernelEntry *t = npdu_pair_table(&cnt);
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].block == B && t[i].ppt == ppt) return &t[i];
+  return nullptr;
+}
+
 int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p) {
   const char *env = std::getenv("FPSX_NPDU_PLAN");
   if (env && std::strncmp(env, "stream", 6) == 0) {
     int g = 0;
     const int got = std::sscanf(env, "stream:%d", &g);
     return plan_npdu_stream(c, n, dim, got == 1 ? (size_t)g : 0, p);
+  }
+  if (env && std::strncmp(env, "pair:", 5) == 0) {
+    int b = 0, q = 0;
+    if (dim != 3 || std::sscanf(env, "pair:%d:%d", &b, &q) != 2)
+      return fail(FPSX_EINVAL, std::string("FPSX_NPDU_PLAN: bad value (pairs are 3-D) ") + env);
+    const KernelEntry *k = find_npdu_pair(b, q);
+    if (!k) return fail(FPSX_EINVAL, std::string("FPSX_NPDU_PLAN: no kernel ") + env);
+    if (plan_resident_shape(c, n, k, p, PlanKind::Npdu) != FPSX_OK || p.grid < 2 ||
+        p.grid > kNpduPairMaxGrid)
+      return fail(FPSX_ETOOBIG, std::string("FPSX_NPDU_PLAN: the paired kernel needs 2..64 workgroups: ") + env);
+    return FPSX_OK;
   }
   if (env && *env) {
     int d = 0, b = 0, q = 0;
@@ -417,9 +436,25 @@ int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p) {
     if (k && plan_resident_shape(c, n, k, p, PlanKind::Npdu) == FPSX_OK) return FPSX_OK;
   }
   // Several workgroups: a pass costs the reduction chain, the window's slots
-  // and a lane scan over PPT slots, plus an exchange growing with G.
+  // and a lane scan over PPT slots, plus an exchange growing with G.  3-D
+  // clouds whose grid fits 2..64 workgroups take the paired form (one
+  // exchange per two picks).  Its round costs about 0.06 us per slot and
+  // barely depends on G (100 K points: 83.9 / 90.2 / 102.3 ms at 4 / 8 / 16
+  // points per thread), so it takes the smallest share that fits 64
+  // workgroups.
   double best = 1e30;
   Plan bp;
+  if (d3 && !(std::getenv("FPSX_NO_PAIR") && std::getenv("FPSX_NO_PAIR")[0] == '1')) {
+    for (int ppt : {2, 4, 8, 16}) {
+      const KernelEntry *k = find_npdu_pair(1024, ppt);
+      Plan q;
+      if (!k || plan_resident_shape(c, n, k, q, PlanKind::Npdu) != FPSX_OK || q.grid < 2 ||
+          q.grid > kNpduPairMaxGrid)
+        continue;
+      p = q;
+      return FPSX_OK;
+    }
+  }
   for (int ppt : {4, 8, 16, 32}) {
     const KernelEntry *k = find_npdu(d3 ? 3 : 0, 1024, ppt);
     Plan q;

@@ -174,8 +174,11 @@ int fpsx_fps_npdu_sampling_device(const float *d_points, size_t stride_point, si
  * Kernel the windowed-update entries would use for a cloud: kernel_dim 3
  * (coordinates in registers) or 0 (any dimension), workgroup size, points per
  * thread and grid.  FPSX_NPDU_PLAN="D:B:PPT" in the environment forces a shape,
- * "stream[:G]" the streaming form (any dimension, min-distances in HBM, used
- * automatically beyond the register file; reported as kernel_dim 0, ppt 0).
+ * "pair:B:PPT" the paired 3-D form (up to two picks per cross-workgroup
+ * exchange, 2..64 workgroups, used automatically where it fits; reported as
+ * kernel_dim 3; FPSX_NO_PAIR=1 keeps the planner off it), "stream[:G]" the
+ * streaming form (any dimension, min-distances in HBM, used automatically
+ * beyond the register file; reported as kernel_dim 0, ppt 0).
  */
 int fpsx_npdu_plan_info(size_t n_points, size_t dim, int *kernel_dim, int *block, int *ppt,
                         int *grid);

@@ -116,6 +116,56 @@ def test_npdu_large_multi_workgroup(gpu, npdu_plan):
         np.testing.assert_array_equal(run(gpu, pc, m, w, 1234), want, err_msg=str(plan))
 
 
+PAIR_SHAPES = ["pair:1024:2", "pair:1024:4", "pair:1024:8", "pair:1024:16"]
+
+
+@pytest.mark.parametrize("shape", PAIR_SHAPES)
+def test_npdu_pair_shapes(gpu, npdu_plan, shape):
+    # Paired form: up to two picks per exchange, same picks as the reference.
+    npdu_plan(shape)
+    rng = np.random.default_rng(46)
+    clouds = [("uniform", rng.random((70000, 3)).astype(np.float32)),
+              ("sorted_gauss", np.sort(rng.standard_normal((60000, 3)).astype(np.float32), axis=0)),
+              ("lattice_dups", np.repeat(rng.integers(0, 6, (12000, 3)), 3, axis=0).astype(np.float32))]
+    for name, pc in clouds:
+        n = len(pc)
+        info = gpu.npdu_plan_info(n, 3)
+        log(f"npdu {shape} {name} plan={info}")
+        assert info["grid"] >= 2
+        for m, w, s in ((1500, 0, 0), (1501, 1, n - 1), (1500, 7, 123), (1200, 64, n // 2),
+                        (1001, 5000, 5), (800, 10 ** 12, 9), (1500, int(n / 1500 * 16), 77)):
+            np.testing.assert_array_equal(run(gpu, pc, m, w, s), O.npdu(pc, m, w, s),
+                                          err_msg=f"{shape} {name} m={m} w={w} s={s}")
+
+
+def test_npdu_pair_special_values(gpu, npdu_plan):
+    npdu_plan("pair:1024:4")
+    rng = np.random.default_rng(47)
+    pc = (rng.random((30000, 3)) * 4e19).astype(np.float32)   # squares overflow to +inf
+    pc[[5, 700, 20000]] = np.nan
+    pc[9] = np.inf
+    log("npdu pair special values")
+    np.testing.assert_array_equal(run(gpu, pc, 600, 32, 0), O.npdu(pc, 600, 32, 0))
+    same = np.ones((20000, 3), np.float32)                      # every distance 0: index order
+    np.testing.assert_array_equal(run(gpu, same, 300, 16, 3), O.npdu(same, 300, 16, 3))
+
+
+def test_npdu_pair_reference_benchmark_size(gpu, npdu_plan):
+    # The reference's own NPDU benchmark size (bench/test_bench.py:151-154):
+    # 100000 points, 50000 picks, w = 32; the paired and the one-pick
+    # kernels agree on every pick and the first 2000 match the oracle.
+    import bench
+
+    pc = bench.gen_uniform_n(100000)
+    npdu_plan("3:1024:8")
+    one = run(gpu, pc, 50000, 32, 0)
+    npdu_plan(None)
+    log(f"npdu 100k auto plan={gpu.npdu_plan_info(100000, 3)}")
+    two = run(gpu, pc, 50000, 32, 0)
+    np.testing.assert_array_equal(two, one)
+    np.testing.assert_array_equal(two[:2000], O.npdu(pc, 2000, 32, 0))
+
+
 def test_npdu_python_api_and_device_entry(gpu):
     import fpsample_amd as F
     from fpsample_amd import device as D
