@@ -1,8 +1,8 @@
 // fps_pruned.hip -- register-resident FPS with per-wave bounding-box pruning.
 //
 // Same picks as fps_resident (vanilla FPS, reference src/lib.cpp:188-246),
-// less work: the points are first put in spatial order (Morton codes, radix
-// sort), and every wave owns one block of 64*PPT consecutive points of that
+// less work: the points are first put in spatial order (Hilbert indices,
+// radix sort), and every wave owns one block of 64*PPT consecutive points of that
 // order, re-sorted inside the block by original index.  A wave keeps the
 // bounding box of its block.  For a pivot p, the squared distance of every
 // point x of the box is >= L(p, box) = sum_j gap_j^2 with gap_j the distance

@@ -1,7 +1,9 @@
 // fpsx_spatial.hip -- spatial order of a 3-D cloud for the pruned FPS kernel.
 //
 //   1. bounding box of the cloud (two-stage min/max, NaN ignored);
-//   2. 30-bit Morton code of every point (10 bits per axis over the box);
+//   2. 30-bit Hilbert index of every point (10 bits per axis over the box;
+//      Hilbert rather than Morton: the curve has no long jumps, so blocks of
+//      consecutive points have smaller boxes and prune better);
 //   3. radix sort of (code, original index)  -> nearby points become neighbours;
 //   4. radix sort of (block of the position, original index): inside every
 //      block of `block` positions the points are re-ordered by original index,
@@ -16,6 +18,7 @@
 
 #include "fps_pruned.hpp"
 #include "fpsx.h"
+#include "fpsx_curve.hpp"
 #include "fpsx_internal.hpp"
 
 namespace fpsx {
@@ -97,21 +100,12 @@ __global__ void __launch_bounds__(kBoxThreads)
   if (threadIdx.x < 6) box[threadIdx.x] = s[threadIdx.x][0];
 }
 
-__device__ __forceinline__ uint32_t spread10(uint32_t v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000ffu;
-  v = (v | (v << 8)) & 0x0300f00fu;
-  v = (v | (v << 4)) & 0x030c30c3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
-__global__ void morton_codes(const float *__restrict__ pts, uint64_t sp, uint64_t ld, uint32_t n,
+__global__ void curve_codes(const float *__restrict__ pts, uint64_t sp, uint64_t ld, uint32_t n,
                              const float *__restrict__ box, uint32_t *__restrict__ keys,
                              uint32_t *__restrict__ vals) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t code = 0;
+  uint32_t qc[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const float lo = box[j], hi = box[3 + j];
@@ -120,9 +114,9 @@ __global__ void morton_codes(const float *__restrict__ pts, uint64_t sp, uint64_
     float q = (pts[(size_t)i * sp + (size_t)j * ld] - lo) * scale;
     if (!(q >= 0.0f)) q = 0.0f;  // NaN and anything below the box
     if (q > 1023.0f) q = 1023.0f;
-    code |= spread10((uint32_t)q) << j;
+    qc[j] = (uint32_t)q;
   }
-  keys[i] = code;
+  keys[i] = hilbert10(qc[0], qc[1], qc[2]);
   vals[i] = i;
 }
 
@@ -183,7 +177,7 @@ int spatial_order(const float *pts, uint64_t sp, uint64_t ld, uint32_t n, uint32
                      (float *)S.box.p);
   HIP_OR_FAIL(hipGetLastError());
   const unsigned g = (n + 255) / 256;
-  hipLaunchKernelGGL(morton_codes, dim3(g), dim3(256), 0, st, pts, sp, ld, n,
+  hipLaunchKernelGGL(curve_codes, dim3(g), dim3(256), 0, st, pts, sp, ld, n,
                      (const float *)S.box.p, (uint32_t *)S.keys.p, (uint32_t *)S.vals.p);
   HIP_OR_FAIL(hipGetLastError());
   HIP_OR_FAIL(hipcub::DeviceRadixSort::SortPairs(S.tmp.p, t1, (uint32_t *)S.keys.p,
