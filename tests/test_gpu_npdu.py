@@ -226,3 +226,21 @@ def test_npdu_beyond_register_file(gpu):
     assert info["ppt"] == 0
     w = int(n / 4096 * 16)
     np.testing.assert_array_equal(run(gpu, pc, 40, w, 123), O.npdu(pc, 40, w, 123))
+
+
+def test_npdu_pair_random_configurations(gpu, npdu_plan):
+    # Randomised: the paired windowed-update kernel (the planner's choice for
+    # 2..64 workgroups) against the one-pick kernel on random sizes, sample
+    # counts, windows and starts.
+    rng = np.random.default_rng(77)
+    for case in range(12):
+        n = int(rng.integers(20_000, 1 << 20))
+        pc = (np.sort(rng.random((n, 3)), axis=0) if case % 2 else rng.random((n, 3))).astype(np.float32)
+        m = int(rng.integers(500, 6000))
+        w = int(rng.choice([0, 1, 7, 33, int(n / m * 16), 5 * n]))
+        s = int(rng.integers(n))
+        npdu_plan("3:1024:8" if n <= 8192 * 1024 else None)
+        one = run(gpu, pc, m, w, s)
+        npdu_plan(None)
+        log(f"npdu random case {case}: n={n} m={m} w={w} s={s} plan={gpu.npdu_plan_info(n, 3)}")
+        np.testing.assert_array_equal(run(gpu, pc, m, w, s), one, err_msg=f"case {case}")

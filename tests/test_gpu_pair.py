@@ -135,3 +135,42 @@ def test_pair_rejects_grids_outside_range(gpu, plan_env):
     pc = np.random.default_rng(45).random((9000, 3)).astype(np.float32)
     rc, _ = gpu.fps(pc, 100, 0)
     assert rc != 0 and "2..64" in gpu.last_error()
+
+
+def test_pair_random_configurations(gpu, plan_env):
+    # Randomised: the paired kernel against the one-pick pruned kernel (itself
+    # checked against the oracle above) on clouds of 70 K - 1 M points,
+    # clustered, uniform, anisotropic and duplicate-heavy, random sample
+    # counts, starts and forced prefixes; the lowest-tie rule through the
+    # bucket entry on a few of them.
+    rng = np.random.default_rng(2024)
+    for case in range(16):
+        n = int(rng.integers(70_000, 1 << 20))
+        kind = case % 4
+        if kind == 0:
+            pc = gauss_cloud(n, int(rng.integers(1 << 30)))
+        elif kind == 1:
+            pc = rng.random((n, 3)).astype(np.float32)
+        elif kind == 2:
+            pc = (rng.standard_normal((n, 3)) * np.array([10.0, 1.0, 0.01])).astype(np.float32)
+        else:
+            sites = rng.integers(0, 64, (n // 4, 3))
+            pc = sites[rng.integers(0, n // 4, n)].astype(np.float32)
+        m = int(rng.integers(1024, 12_000))
+        starts = int(rng.integers(n)) if case % 3 else [int(x) for x in rng.integers(0, n, 5)]
+        plan_env("pruned:1024:16")
+        one = run_fps(gpu, pc, m, starts)
+        plan_env(None)
+        info = gpu.plan_info(n, 3, m=m)
+        log(f"random case {case}: n={n} kind={kind} m={m} starts={starts} plan={info}")
+        assert info["kind"] == 5, info
+        np.testing.assert_array_equal(run_fps(gpu, pc, m, starts), one, err_msg=f"case {case}")
+        if case % 4 == 1:
+            h = int(rng.integers(3, 10))
+            plan_env("pruned:1024:16")
+            rc, want = gpu.kdline(pc, 2000, h, 11)
+            assert rc == 0, gpu.last_error()
+            plan_env(None)
+            rc, got = gpu.kdline(pc, 2000, h, 11)
+            assert rc == 0, gpu.last_error()
+            np.testing.assert_array_equal(got, want, err_msg=f"bucket case {case}")
