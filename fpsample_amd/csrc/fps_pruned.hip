@@ -764,17 +764,30 @@ __global__ void __launch_bounds__(1024) fps_stream_pruned(PrunedParams a) {
         float bc[D] = {0.0f, 0.0f, 0.0f};
         float lo[D] = {INFINITY, INFINITY, INFINITY};
         float hi[D] = {-INFINITY, -INFINITY, -INFINITY};
-        // Fully unrolled, loads unconditional (a position past the cloud
-        // reads the last point and is masked out), so every load of the
-        // block is in flight at once: one memory round trip per block.
+        // Every load of the block first, then the updates and their stores:
+        // a store between two slots' loads would keep the later loads behind
+        // it (the compiler cannot rule out aliasing), one memory round trip
+        // per slot.  Loads are unconditional (a position past the cloud reads
+        // the last point and is masked out).
+        float bx[SL][D], bo[SL];
+        uint32_t bid[SL];
+#pragma unroll
+        for (int k = 0; k < SL; ++k) {
+          const uint32_t p = base + (uint32_t)(k * 64 + lane);
+          const uint32_t q = p < n ? p : n - 1;
+          bx[k][0] = xs[q];
+          bx[k][1] = ys[q];
+          bx[k][2] = zs[q];
+          bid[k] = a.perm[q];
+          bo[k] = (s == 1) ? f.init : f.dmin[q];
+        }
 #pragma unroll
         for (int k = 0; k < SL; ++k) {
           const uint32_t p = base + (uint32_t)(k * 64 + lane);
           const bool live = p < n;
-          const uint32_t q = live ? p : n - 1;
-          const float c[D] = {xs[q], ys[q], zs[q]};
-          const uint32_t id = a.perm[q];
-          const float old = (s == 1) ? f.init : f.dmin[q];
+          const float(&c)[D] = bx[k];
+          const uint32_t id = bid[k];
+          const float old = bo[k];
           const float nm = min_keep(sqdist<D>(c, pv), old);
           if (live && (s == 1 || __float_as_int(nm) != __float_as_int(old))) f.dmin[p] = nm;
           const int b = __float_as_int(nm);
