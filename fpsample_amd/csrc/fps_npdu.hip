@@ -639,6 +639,7 @@ __global__ void __launch_bounds__(1024) fps_npdu_stream(NpduParams a) {
   if (recorder) record(0);
 
   const int64_t hw = (int64_t)(a.window / 2);
+  const size_t lane_off = (size_t)lane * f.sp;
   uint32_t e = 0;
   for (uint32_t s = 1; s < f.m; ++s) {
     uint32_t lo = 0, hi = n - 1;
@@ -663,26 +664,64 @@ __global__ void __launch_bounds__(1024) fps_npdu_stream(NpduParams a) {
     const uint32_t j_end = (bhi >= g) ? min(nbl, (bhi - g) / G + 1) : 0u;
     for (uint32_t j = j_first + (uint32_t)wave; j < j_end; j += W) {
       const uint32_t base = (g + j * G) * (uint32_t)kNpduBlock;
+      // Every load of the block before any update or store: a store between
+      // two slots' loads would hold the later loads behind it (the compiler
+      // cannot rule out aliasing), one memory round trip per slot and
+      // dimension.  Here a block costs one round trip for its min-distances
+      // and one per dimension, each of SL independent loads.
+      // In the window: lo <= p <= hi (hi < n, so p is a point of the cloud).
+      const uint32_t pl = base + (uint32_t)lane;
+      const uint32_t span = hi - lo;
+      auto in_window = [&](int k) { return pl + (uint32_t)(k * 64) - lo <= span; };
+      float old[SL], acc[SL];
+#pragma unroll
+      for (int k = 0; k < SL; ++k) {
+        const uint32_t p = pl + (uint32_t)(k * 64);
+        old[k] = (s == 1) ? f.init : f.dmin[p < n ? p : n - 1];
+        acc[k] = 0.0f;
+      }
+      // dimension by dimension in the reference's order (dist = 0;
+      // dist += d*d; and 0 + x == x for every x >= 0 and for NaN); only
+      // in-window positions read their coordinates.
+      for (uint32_t c = 0; c < dim; ++c) {
+        const float pvc = s_pv[c];
+        // A running pointer over the slots' rows (base + 64 k + lane) and
+        // the window test formed afresh per dimension: both are opaque to
+        // the compiler, so that neither 16 per-slot addresses nor 16 masks
+        // stay live across the loop (they spill, and address temporaries
+        // that reuse the registers of loads in flight add waits).
+        const float *rows = f.pts + (size_t)base * f.sp + (size_t)c * f.ld;
+        const size_t step = (size_t)64 * f.sp;
+        size_t off = lane_off;
+        uint32_t plc = pl;
+        asm volatile("" : "+v"(plc));
+        float x[SL];
+#pragma unroll
+        for (int k = 0; k < SL; ++k) {
+          x[k] = (plc + (uint32_t)(k * 64) - lo <= span) ? rows[off] : 0.0f;
+          off += step;
+          asm volatile("" : "+v"(off));
+        }
+#pragma unroll
+        for (int k = 0; k < SL; ++k) {
+          const float dc = x[k] - pvc;
+          acc[k] = (c == 0) ? dc * dc : acc[k] + dc * dc;
+        }
+      }
+      // One wait for the min-distance loads here; otherwise every slot's
+      // update below waits for all memory operations, the previous slot's
+      // store included.
+      settle<SL>(old);
       int bv = INT_MIN;
       uint32_t bi = 0;
 #pragma unroll
       for (int k = 0; k < SL; ++k) {
-        const uint32_t p = base + (uint32_t)(k * 64 + lane);
+        const uint32_t p = pl + (uint32_t)(k * 64);
         const bool live = p < n;
-        const uint32_t q = live ? p : n - 1;
-        const float old = (s == 1) ? f.init : f.dmin[q];
-        float nm = old;
-        if (live && p >= lo && p <= hi) {
-          // dimension by dimension in the reference's order (dist = 0;
-          // dist += d*d; and 0 + x == x for every x >= 0 and for NaN)
-          const float *pp = f.pts + (size_t)p * f.sp;
-          float acc = 0.0f;
-          for (uint32_t c = 0; c < dim; ++c) {
-            const float dc = pp[(size_t)c * f.ld] - s_pv[c];
-            acc = (c == 0) ? dc * dc : acc + dc * dc;
-          }
-          nm = min_keep(acc, old);
-          if (s == 1 || __float_as_int(nm) != __float_as_int(old)) f.dmin[p] = nm;
+        float nm = old[k];
+        if (in_window(k)) {
+          nm = min_keep(acc[k], old[k]);
+          if (s == 1 || __float_as_int(nm) != __float_as_int(old[k])) f.dmin[p] = nm;
         }
         // Strict `>` over ascending indices: the lowest index among equals.
         const int b = __float_as_int(nm);
