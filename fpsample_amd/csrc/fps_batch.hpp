@@ -42,6 +42,7 @@ struct BatchParams {
   float init;                // initial min-distance
   uint64_t spin_ticks;       // bound on one exchange wait
   uint64_t team_ticks;       // bound on waiting for a team to form
+  uint32_t pair;             // 1: teams settle up to two picks per exchange
 };
 
 const KernelEntry *batch_table(size_t *count);

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -128,6 +129,10 @@ int BatchRun::prepare(Ctx &c, const KernelEntry *kernel, const std::vector<Batch
   params.init = INFINITY;
   params.spin_ticks = c.spin_ticks;
   params.team_ticks = c.spin_ticks * 5;
+  // Teams settle up to two picks per exchange unless FPSX_BATCH_NO_PAIR=1
+  // (the one-pick team loop; same picks).
+  const char *np = std::getenv("FPSX_BATCH_NO_PAIR");
+  params.pair = (np && np[0] == '1') ? 0u : 1u;
   return FPSX_OK;
 }
 

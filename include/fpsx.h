@@ -194,6 +194,8 @@ int fpsx_npdu_plan_info(size_t n_points, size_t dim, int *kernel_dim, int *block
  *   FPSX_BATCH_GRAPH     capture the launch sequence in a HIP graph, replayed by run
  *   FPSX_BATCH_AUTOTUNE  time every batch-kernel shape once at creation, keep the fastest
  * Clouds of a dimension with no batch kernel run one launch per cloud.
+ * Clouds that need several workgroups settle up to two picks per exchange
+ * (same picks); FPSX_BATCH_NO_PAIR=1, read at creation, keeps one per exchange.
  */
 #define FPSX_BATCH_GRAPH 1
 #define FPSX_BATCH_AUTOTUNE 2

@@ -108,3 +108,44 @@ def test_batch_oversize_cloud_falls_back(gpu):
         np.testing.assert_array_equal(g, w)
     b = Bt.DeviceBatch(clouds, ms, [2, 2, 2])
     assert b.info()["block"] == 0  # the per-cloud form
+
+
+def test_batch_paired_teams_match_one_pick(gpu, monkeypatch):
+    # Teams settle up to two picks per exchange; FPSX_BATCH_NO_PAIR=1 runs the
+    # one-pick team loop.  256:4 holds 1024 points per workgroup, so clouds of
+    # 2k..60k form teams of 2..59: random clouds (uniform, sphere, integer
+    # ties, all points equal, nan/inf/huge coordinates), odd and even sample
+    # counts, random starts; both loops and the oracle must agree.
+    import fpsample_amd as F
+
+    monkeypatch.setenv("FPSX_BATCH_SHAPE", "256:4")
+    rng = np.random.default_rng(31)
+    clouds, ms, starts = [], [], []
+    for i in range(18):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(60000))))
+        kind = i % 6
+        if kind == 0:
+            pc = rng.random((n, 3))
+        elif kind == 1:
+            v = rng.standard_normal((n, 3))
+            pc = v / np.linalg.norm(v, axis=1, keepdims=True)
+        elif kind == 2:
+            pc = rng.integers(0, 4, (n, 3))
+        elif kind == 3:
+            pc = np.tile(rng.random((1, 3)), (n, 1))
+        elif kind == 4:
+            pc = rng.random((n, 3)) * 4e19
+            pc[rng.integers(0, n, 3)] = np.nan
+            pc[rng.integers(0, n), 1] = np.inf
+        else:
+            pc = rng.standard_normal((n, 3)) * np.array([100.0, 1.0, 1e-3])
+        clouds.append(pc.astype(np.float32))
+        ms.append(int(rng.integers(2, min(n, 3000))) | (i & 1))
+        starts.append(int(rng.integers(0, n)))
+    log(f"paired teams: sizes {[len(c) for c in clouds]} picks {ms}")
+    paired = F.fps_sampling_batch(clouds, ms, starts)
+    monkeypatch.setenv("FPSX_BATCH_NO_PAIR", "1")
+    one = F.fps_sampling_batch(clouds, ms, starts)
+    for i, (p, o, w) in enumerate(zip(paired, one, oracle_all(clouds, ms, starts))):
+        np.testing.assert_array_equal(o, w, err_msg=f"one-pick cloud {i} n={len(clouds[i])}")
+        np.testing.assert_array_equal(p, w, err_msg=f"paired cloud {i} n={len(clouds[i])}")

@@ -71,3 +71,19 @@ def test_serve_same_alone_and_in_mix(gpu):
         mixed = [srv.wait(r)[0] for r in ids]
     for a, b in zip(alone, mixed):
         np.testing.assert_array_equal(a, b)
+
+
+def test_serve_short_chunks_paired_teams(gpu):
+    # Chunks of 7 passes: paired rounds end on both sides of chunk boundaries
+    # (a chunk ending on a pair saves its state with the pair's first pick
+    # applied).  The 256:8 kernel (2048 points per workgroup; the default
+    # 1024:16 one keeps the one-pick loop) runs 10k..60k-point requests in
+    # teams of 5..30.
+    from fpsample_amd.serve import Server
+
+    reqs = requests(10, 8, 10000, 60000)
+    with Server(hbm_budget_bytes=512 << 20, chunk_passes=7, block=256, ppt=8) as srv:
+        ids = [srv.submit(pc, min(m, 700), s) for pc, m, s in reqs]
+        outs = [srv.wait(r)[0] for r in ids]
+    for i, ((pc, m, s), got) in enumerate(zip(reqs, outs)):
+        np.testing.assert_array_equal(got, O.fps(pc, min(m, 700), s), err_msg=f"request {i}")
