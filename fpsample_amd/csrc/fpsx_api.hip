@@ -290,16 +290,29 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
 
   if (plan_from_tuned(c, n, dim, tie, p) == FPSX_OK) return FPSX_OK;
 
-  // Large 3-D clouds with many samples: spatial order + per-wave pruning.
-  // The ordering costs a few radix-sort passes over the cloud, repaid once
-  // most passes touch few waves; the shape holding the most points per
-  // workgroup keeps the exchange small.
+  // 3-D clouds with enough samples: spatial order + per-wave pruning.  The
+  // ordering costs a few radix-sort passes over the cloud, repaid once most
+  // passes touch few waves.  Measured on MI355X (uniform and clustered
+  // clouds, whole call): up to 16 K points one pruned workgroup (no
+  // exchange) with the smallest share holding the cloud beats the resident
+  // kernel from 4 K points and 1024 samples (16 K points, 4096 samples:
+  // 4.6 ms against 7.2 ms); beyond, the paired form with the smallest share
+  // its 64-workgroup limit allows (64 K points, 16384 samples: 24.3 ms at
+  // 4 points per thread against 29.1 ms at 16).
   const char *np = std::getenv("FPSX_NO_PRUNE");
   const char *npair = std::getenv("FPSX_NO_PAIR");
-  if (dim == 3 && n >= 65536 && m >= 1024 && !(np && np[0] == '1')) {
+  const bool prune = dim == 3 && !(np && np[0] == '1');
+  if (prune && n <= 16384 && ((n >= 4096 && m >= 1024) || (n > 4096 && m >= 512))) {
+    for (int ppt : {4, 8, 16}) {
+      if ((size_t)1024 * ppt < n) continue;
+      const KernelEntry *k = find_pruned(3, 1024, ppt, tie);
+      if (k && plan_resident_shape(c, n, k, p, PlanKind::Pruned) == FPSX_OK) return FPSX_OK;
+    }
+  }
+  if (prune && n > 16384 && m >= 1024) {
     // Paired form first (half the exchanges) where its grid limit allows.
     if (!(npair && npair[0] == '1')) {
-      static const int kPa[][2] = {{1024, 16}, {1024, 8}, {1024, 4}};
+      static const int kPa[][2] = {{1024, 4}, {1024, 8}, {1024, 16}};
       for (auto &sh : kPa) {
         const KernelEntry *k = find_pair(3, sh[0], sh[1], tie);
         if (k && plan_resident_shape(c, n, k, p, PlanKind::Pair) == FPSX_OK && p.grid >= 2 &&
@@ -307,10 +320,13 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
           return FPSX_OK;
       }
     }
+    // One pick per exchange (beyond 64 workgroups, or FPSX_NO_PAIR=1): from
+    // 64 K points, where it beats the resident kernel.
     static const int kPr[][2] = {{1024, 16}, {256, 32}, {1024, 8}, {1024, 4}};
     for (auto &sh : kPr) {
       const KernelEntry *k = find_pruned(3, sh[0], sh[1], tie);
-      if (k && plan_resident_shape(c, n, k, p, PlanKind::Pruned) == FPSX_OK) return FPSX_OK;
+      if (n >= 65536 && k && plan_resident_shape(c, n, k, p, PlanKind::Pruned) == FPSX_OK)
+        return FPSX_OK;
     }
   }
 

@@ -264,13 +264,43 @@ def test_pruned_auto_plan_and_bucket_order(gpu):
     pc = c2_cloud(120000, seed=5)
     info = gpu.plan_info(len(pc), 3, m=2048)
     log(f"auto plan {info}")
-    assert info["kind"] == 5  # paired pruned kernel (8 workgroups)
+    assert info["kind"] == 5 and info["ppt"] == 4  # paired pruned kernel, 30 workgroups
     np.testing.assert_array_equal(run_fps(gpu, pc, 2048, 77), O.fps(pc, 2048, 77))
     # bucket entry: lowest-position ties over the KD-line order, pruned plan
     lat = rng.integers(0, 40, (100000, 3)).astype(np.float32)
     rc, out = gpu.kdline(lat, 1500, 123, 7)
     assert rc == 0, gpu.last_error()
     np.testing.assert_array_equal(out, O.kdline_quickfps(lat, 1500, 7, 123))
+
+
+def test_small_clouds_auto_plans(gpu):
+    # The planner's choices below 64 K points: one pruned workgroup (kind 2,
+    # grid 1, the smallest share holding the cloud) from 4 K points and 1024
+    # samples or 8 K points and 512 samples; the paired kernel at 4 points per
+    # thread above 16 K points; the resident kernel otherwise.  Every choice
+    # against the oracle, both tie rules (the bucket entry ties lowest).
+    rng = np.random.default_rng(35)
+    cases = [(4096, 1024, 2, 4), (5000, 600, 2, 8), (8192, 2048, 2, 8), (12000, 3000, 2, 16),
+             (16384, 4096, 2, 16), (16385, 1024, 5, 4), (40000, 2500, 5, 4),
+             (4096, 256, 0, None), (3000, 1500, 0, None), (30000, 500, 0, None)]
+    for n, m, kind, ppt in cases:
+        info = gpu.plan_info(n, 3, m=m)
+        assert info["kind"] == kind and (ppt is None or info["ppt"] == ppt), (n, m, info)
+        for name, pc in (("uniform", rng.random((n, 3)).astype(np.float32)),
+                         ("gauss", c2_cloud(n, seed=n)),
+                         ("dups", rng.integers(0, 5, (n, 3)).astype(np.float32))):
+            s0 = int(rng.integers(n))
+            log(f"small auto n={n} m={m} {name} start={s0} plan={info}")
+            np.testing.assert_array_equal(run_fps(gpu, pc, m, s0), O.fps(pc, m, s0),
+                                          err_msg=f"n={n} m={m} {name}")
+    pc = (rng.random((12000, 3)) * 4e19).astype(np.float32)
+    pc[[5, 700, 4000]] = np.nan
+    pc[9] = np.inf
+    np.testing.assert_array_equal(run_fps(gpu, pc, 3000, 0), O.fps(pc, 3000, 0))
+    lat = rng.integers(0, 12, (12000, 3)).astype(np.float32)
+    rc, out = gpu.kdline(lat, 3000, 5, 7)
+    assert rc == 0, gpu.last_error()
+    np.testing.assert_array_equal(out, O.kdline_quickfps(lat, 3000, 7, 5))
 
 
 def test_bucket_entry_full_size_matches_oracle(gpu):
