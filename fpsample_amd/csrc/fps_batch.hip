@@ -381,8 +381,6 @@ __global__ void __launch_bounds__(B) fps_batch(BatchParams a) {
   constexpr int NW = 2 + D;
   __shared__ int s_cand[2][NW][W];
   __shared__ int s_xw[2][NW + 1][W];
-  __shared__ int s_la[2][NW][W];
-  __shared__ int s_res[2][2 * NW + 2];
   __shared__ uint32_t s_claim;
   __shared__ int s_ok;
   const int tid = threadIdx.x;
@@ -418,17 +416,23 @@ __global__ void __launch_bounds__(B) fps_batch(BatchParams a) {
     }
     // The paired loop holds a second pivot and the lookahead's candidate:
     // 1024-thread workgroups (128 VGPRs) fit it up to 8 points per thread.
-    constexpr bool kPairFits = B < 1024 || PPT <= 8;
-    bool done;
-    if constexpr (kPairFits) {
-      done = (a.pair && J.team > 1) ? run_job_pair<D, B, PPT, TIE>(a, J, r, s_cand, s_la, s_res)
-                                    : run_job<D, B, PPT, TIE>(a, J, j, r, s_cand, s_xw);
+    // Other shapes compile the one-pick loop alone (its code and LDS exactly
+    // as without the paired loop).
+    if constexpr (B < 1024 || PPT <= 8) {
+      __shared__ int s_la[2][NW][W];
+      __shared__ int s_res[2][2 * NW + 2];
+      const bool done = (a.pair && J.team > 1)
+                            ? run_job_pair<D, B, PPT, TIE>(a, J, r, s_cand, s_la, s_res)
+                            : run_job<D, B, PPT, TIE>(a, J, j, r, s_cand, s_xw);
+      if (!done) {
+        if (tid == 0) atomicOr(a.status, kStatusTimeout);
+        return;
+      }
     } else {
-      done = run_job<D, B, PPT, TIE>(a, J, j, r, s_cand, s_xw);
-    }
-    if (!done) {
-      if (tid == 0) atomicOr(a.status, kStatusTimeout);
-      return;
+      if (!run_job<D, B, PPT, TIE>(a, J, j, r, s_cand, s_xw)) {
+        if (tid == 0) atomicOr(a.status, kStatusTimeout);
+        return;
+      }
     }
     lds_barrier();  // every lane has read s_claim / s_ok before they are rewritten
   }

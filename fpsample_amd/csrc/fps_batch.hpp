@@ -40,9 +40,9 @@ struct BatchParams {
   uint64_t *xchg;            // team exchange areas (kTeamXchgWords each), zeroed
   uint32_t *status;          // status word, zeroed
   float init;                // initial min-distance
+  uint32_t pair;             // 1: teams settle up to two picks per exchange
   uint64_t spin_ticks;       // bound on one exchange wait
   uint64_t team_ticks;       // bound on waiting for a team to form
-  uint32_t pair;             // 1: teams settle up to two picks per exchange
 };
 
 const KernelEntry *batch_table(size_t *count);
