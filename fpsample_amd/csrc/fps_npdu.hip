@@ -774,11 +774,176 @@ __global__ void __launch_bounds__(1024) fps_npdu_stream(NpduParams a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// 3-D streaming form: fps_npdu_stream with each block's best coordinates
+// cached in LDS beside its (value, index).  The workgroup's winner then
+// travels with its coordinates through the exchange, and no workgroup reads
+// the pivot back from memory: one memory round trip and one barrier less per
+// pass.  A block update loads the coordinates of every position of the block
+// (in the same batch as its min-distances), since its best may lie outside
+// the window.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) fps_npdu_stream3(NpduParams a) {
+  constexpr int B = 1024;
+  constexpr int W = B / 64;
+  constexpr int D = 3;
+  constexpr int NW = 2 + D;
+  constexpr int SL = kNpduBlock / 64;
+  __shared__ int s_cand[2][NW][W];
+  __shared__ int s_xw[2][NW + 1][W];
+  extern __shared__ int s_dyn[];  // block values (nbl), indices (nbl), coordinates (3 nbl)
+
+  const FpsParams &f = a.fps;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const uint32_t G = gridDim.x;
+  const uint32_t g = blockIdx.x;
+  const uint32_t n = f.n;
+  const uint32_t nblocks = (n + kNpduBlock - 1) / kNpduBlock;
+  const uint32_t nbl = (nblocks > g) ? (nblocks - g + G - 1) / G : 0u;
+  int *m_v = s_dyn;
+  int *m_i = m_v + nbl;
+  int *m_c = m_i + nbl;  // m_c[c * nbl + j]: coordinate bits of block j's best
+
+  uint64_t st = f.starts[0];
+  if (st >= n) {
+    if (tid == 0) atomicOr(f.status, kStatusBadStart);
+    st = 0;
+  }
+  uint32_t pidx = (uint32_t)st;
+  float pv[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) pv[c] = f.pts[(size_t)pidx * f.sp + (size_t)c * f.ld];
+  settle<D>(pv);
+  const bool recorder = (g == 0 && tid >= B - 64);
+  if (recorder) record_pick<D>(f, 0, pidx, pv);
+
+  const int64_t hw = (int64_t)(a.window / 2);
+  uint32_t e = 0;
+  for (uint32_t s = 1; s < f.m; ++s) {
+    uint32_t lo = 0, hi = n - 1;
+    if (s > 1) {
+      int64_t l = (int64_t)pidx - hw;
+      int64_t h = (int64_t)pidx + hw;
+      if (l < 0) {
+        h -= l;
+        l = 0;
+      }
+      if (h >= (int64_t)n) {
+        l = l - (h - (int64_t)n + 1);
+        if (l < 0) l = 0;
+        h = (int64_t)n - 1;
+      }
+      lo = (uint32_t)l;
+      hi = (uint32_t)h;
+    }
+    const uint32_t blo = lo / kNpduBlock, bhi = hi / kNpduBlock;
+    const uint32_t j_first = (blo > g) ? (blo - g + G - 1) / G : 0u;
+    const uint32_t j_end = (bhi >= g) ? min(nbl, (bhi - g) / G + 1) : 0u;
+    for (uint32_t j = j_first + (uint32_t)wave; j < j_end; j += W) {
+      const uint32_t base = (g + j * G) * (uint32_t)kNpduBlock;
+      // Every load of the block (min-distances and coordinates, positions
+      // past the cloud clamped to the last point and masked out) before any
+      // update or store: one memory round trip per block.
+      const uint32_t pl = base + (uint32_t)lane;
+      const uint32_t span = hi - lo;  // in the window: p - lo <= span (hi < n)
+      float old[SL], x[SL][D];
+#pragma unroll
+      for (int k = 0; k < SL; ++k) {
+        const uint32_t p = pl + (uint32_t)(k * 64);
+        const uint32_t q = p < n ? p : n - 1;
+        old[k] = (s == 1) ? f.init : f.dmin[q];
+        const float *pp = f.pts + (size_t)q * f.sp;
+#pragma unroll
+        for (int c = 0; c < D; ++c) x[k][c] = pp[(size_t)c * f.ld];
+      }
+      int bv = INT_MIN;
+      uint32_t bi = 0;
+      float bc[D] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int k = 0; k < SL; ++k) {
+        const uint32_t p = pl + (uint32_t)(k * 64);
+        const bool live = p < n;
+        float nm = old[k];
+        if (p - lo <= span) {
+          nm = min_keep(sqdist<D>(x[k], pv), old[k]);
+          if (s == 1 || __float_as_int(nm) != __float_as_int(old[k])) f.dmin[p] = nm;
+        }
+        // Strict `>` over ascending indices: the lowest index among equals.
+        const int b = __float_as_int(nm);
+        const bool take = live && b > bv;
+        bv = take ? b : bv;
+        bi = take ? p : bi;
+#pragma unroll
+        for (int c = 0; c < D; ++c) bc[c] = take ? x[k][c] : bc[c];
+      }
+      int wv;
+      const int wl = wave_argmax_lane<kTieLowest>(bv, bi, wv);
+      const int wi = readlane_i((int)bi, wl);
+      float wc[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) wc[c] = readlane_f(bc[c], wl);
+      if (lane == 0) {
+        m_v[j] = wv;
+        m_i[j] = wi;
+#pragma unroll
+        for (int c = 0; c < D; ++c) m_c[c * nbl + j] = __float_as_int(wc[c]);
+      }
+    }
+    lds_barrier();  // the cached bests are current
+    ++e;
+    const int par = (int)(e & 1u);
+    int bv = INT_MIN;
+    uint32_t bi = 0;
+    uint32_t bj = 0;
+    for (uint32_t j = (uint32_t)lane; j < nbl; j += 64u) {
+      const int v = m_v[j];
+      const uint32_t id = (uint32_t)m_i[j];
+      if (beats<kTieLowest>(v, id, bv, bi)) {
+        bv = v;
+        bi = id;
+        bj = j;
+      }
+    }
+    int gv;
+    const int gl = wave_argmax_lane<kTieLowest>(bv, bi, gv);
+    const uint32_t wj = (uint32_t)readlane_i((int)bj, gl);
+    int win[NW];
+    win[0] = gv;
+    win[1] = readlane_i((int)bi, gl);
+#pragma unroll
+    for (int c = 0; c < D; ++c) win[2 + c] = m_c[c * nbl + wj];
+    if (G == 1) {
+      lds_barrier();  // every wave is done reading the cache before the next update
+    } else {
+      if (wave == 0 && lane < NW) {
+        int w = win[0];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) w = (lane == q) ? win[q] : w;
+        s_cand[par][lane][0] = w;
+      }
+      // The exchange publishes row 0 of s_cand from wave 0 (which just wrote
+      // it) and ends with a workgroup barrier.
+      if (!exchange<kTieLowest, NW, W>(f.xchg, f.replicas, kPageWords, G, g, e, f.backoff,
+                                       f.spin_ticks, s_cand[par], s_xw[par], 0, win)) {
+        if (tid == 0) atomicOr(f.status, kStatusTimeout);
+        return;
+      }
+    }
+    pidx = (uint32_t)win[1];
+#pragma unroll
+    for (int c = 0; c < D; ++c) pv[c] = __int_as_float(win[2 + c]);
+    if (recorder) record_pick<D>(f, s, pidx, pv);
+  }
+}
+
 static const KernelEntry kNpduStream[] = {
     {0, 1024, 0, kTieLowest, (const void *)&fps_npdu_stream},
+    {3, 1024, 0, kTieLowest, (const void *)&fps_npdu_stream3},
 };
 
-const KernelEntry *npdu_stream_kernel() { return &kNpduStream[0]; }
+const KernelEntry *npdu_stream_kernel(bool three_d) { return &kNpduStream[three_d ? 1 : 0]; }
 
 #define FPSX_NPDU(D, B, P) {D, B, P, kTieLowest, (const void *)&fps_npdu<D, B, P>}
 

@@ -23,6 +23,8 @@ constexpr uint32_t kNpduPairMaxGrid = 64;
 // Streaming form (any dimension, ppt 0): blocks of kNpduBlock indices; LDS
 // holds dim floats of pivot plus 2 words per block of the workgroup.
 constexpr int kNpduBlock = 1024;
-const KernelEntry *npdu_stream_kernel();
+// Streaming windowed-update kernel: any dimension, or the 3-D form that
+// carries the winner's coordinates through the exchange.
+const KernelEntry *npdu_stream_kernel(bool three_d);
 
 }  // namespace fpsx

@@ -381,15 +381,19 @@ static const KernelEntry *find_npdu(int dim, int B, int ppt) {
 // suffices (16384 / 32768 points); beyond, 1024-thread workgroups with the
 // share per thread that minimises 0.012 us per slot + 0.0125 us per
 // workgroup of exchange (fitted to MI355X sweeps at 50 K and 100 K points).
-// Streaming windowed-update plan (any dimension): G chosen so that a
-// workgroup caches about 256 blocks; want_g > 0 forces G.
-static int plan_npdu_stream(Ctx &c, size_t n, size_t dim, size_t want_g, Plan &p) {
-  const KernelEntry *k = npdu_stream_kernel();
+// Streaming windowed-update plan: G chosen so that a workgroup caches about
+// 256 blocks; want_g > 0 forces G.  3-D clouds take the kernel that carries
+// the winner's coordinates (5 LDS words per block) unless any_dim is set;
+// the any-dimension kernel keeps the pivot (dim words) and 2 words per block.
+static int plan_npdu_stream(Ctx &c, size_t n, size_t dim, size_t want_g, Plan &p,
+                            bool any_dim = false) {
+  const KernelEntry *k = npdu_stream_kernel(dim == 3 && !any_dim);
   const size_t nblocks = (n + kNpduBlock - 1) / kNpduBlock;
   size_t g = want_g ? want_g : std::max<size_t>(1, (nblocks + 255) / 256);
   g = std::min<size_t>(g, std::min<size_t>(nblocks, 256));
   const size_t nbl = (nblocks + g - 1) / g;
-  const size_t shmem = dim * sizeof(float) + 2 * nbl * sizeof(int);
+  const size_t shmem = (k->dim == 3) ? 5 * nbl * sizeof(int)
+                                     : dim * sizeof(float) + 2 * nbl * sizeof(int);
   if (shmem > 96 * 1024)
     return fail(FPSX_ETOOBIG, "windowed-update streaming plan: too many blocks per workgroup");
   const size_t gmax = (size_t)c.max_blocks(k, shmem);
@@ -415,9 +419,11 @@ static const KernelEntry *find_npdu_pair(int B, int ppt) {
 int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p) {
   const char *env = std::getenv("FPSX_NPDU_PLAN");
   if (env && std::strncmp(env, "stream", 6) == 0) {
+    // "stream[:G]"; "stream0[:G]" forces the any-dimension kernel.
+    const bool any = std::strncmp(env, "stream0", 7) == 0;
     int g = 0;
-    const int got = std::sscanf(env, "stream:%d", &g);
-    return plan_npdu_stream(c, n, dim, got == 1 ? (size_t)g : 0, p);
+    const int got = std::sscanf(env + (any ? 7 : 6), ":%d", &g);
+    return plan_npdu_stream(c, n, dim, got == 1 ? (size_t)g : 0, p, any);
   }
   if (env && std::strncmp(env, "pair:", 5) == 0) {
     int b = 0, q = 0;

@@ -177,8 +177,10 @@ int fpsx_fps_npdu_sampling_device(const float *d_points, size_t stride_point, si
  * "pair:B:PPT" the paired 3-D form (up to two picks per cross-workgroup
  * exchange, 2..64 workgroups, used automatically where it fits; reported as
  * kernel_dim 3; FPSX_NO_PAIR=1 keeps the planner off it), "stream[:G]" the
- * streaming form (any dimension, min-distances in HBM, used automatically
- * beyond the register file; reported as kernel_dim 0, ppt 0).
+ * streaming form (min-distances in HBM, used automatically beyond the
+ * register file; ppt 0, kernel_dim 3 for 3-D clouds, whose winners carry
+ * their coordinates through the exchange, else 0), "stream0[:G]" its
+ * any-dimension kernel on any cloud.
  */
 int fpsx_npdu_plan_info(size_t n_points, size_t dim, int *kernel_dim, int *block, int *ppt,
                         int *grid);

@@ -200,9 +200,12 @@ def test_npdu_error_codes(gpu):
     assert gpu.npdu(pc, 3, 4, 0, dim=0)[0] == 1
 
 
-@pytest.mark.parametrize("plan", ["stream", "stream:1", "stream:4", "stream:13"])
+@pytest.mark.parametrize("plan", ["stream", "stream:1", "stream:4", "stream:13", "stream0",
+                                  "stream0:4"])
 def test_npdu_streaming_form(gpu, npdu_plan, plan):
-    # Min-distances in HBM, per-block cached bests, window blocks only.
+    # Min-distances in HBM, per-block cached bests, window blocks only.  3-D
+    # clouds take the kernel whose winners carry their coordinates; "stream0"
+    # forces the any-dimension kernel on them too.
     npdu_plan(plan)
     rng = np.random.default_rng(47)
     for name, pc, w in (("uniform3", rng.random((20000, 3)).astype(np.float32), 40),
@@ -223,7 +226,7 @@ def test_npdu_beyond_register_file(gpu):
     pc = rng.random((n, 3)).astype(np.float32)
     info = gpu.npdu_plan_info(n, 3)
     log(f"npdu 9M plan {info}")
-    assert info["ppt"] == 0
+    assert info["ppt"] == 0 and info["kernel_dim"] == 3
     w = int(n / 4096 * 16)
     np.testing.assert_array_equal(run(gpu, pc, 40, w, 123), O.npdu(pc, 40, w, 123))
 
