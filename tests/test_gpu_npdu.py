@@ -247,3 +247,27 @@ def test_npdu_pair_random_configurations(gpu, npdu_plan):
         npdu_plan(None)
         log(f"npdu random case {case}: n={n} m={m} w={w} s={s} plan={gpu.npdu_plan_info(n, 3)}")
         np.testing.assert_array_equal(run(gpu, pc, m, w, s), one, err_msg=f"case {case}")
+
+
+@pytest.mark.parametrize("plan", ["stream", "stream0:3", "pair:1024:2"])
+def test_npdu_fused_gather_plans(gpu, npdu_plan, plan):
+    # The 3-D streaming kernel writes the picks' coordinates from the
+    # exchanged winner words, the any-dimension one from its pivot in LDS,
+    # the paired kernel for both picks of a round.
+    from fpsample_amd import device as D
+
+    npdu_plan(plan)
+    rng = np.random.default_rng(50)
+    n, m = 9000, 701
+    q = rng.standard_normal((n, 3)).astype(np.float32)
+    rows = D.DeviceArray.from_numpy(q)
+    ds = D.DeviceArray.from_numpy(np.array([77], np.uint64))
+    out = D.DeviceArray((m,), np.uint64)
+    opts = D.DeviceArray((m, 3), np.float32)
+    log(f"npdu fused gather {plan}")
+    D.fps_npdu_sampling_device(rows, 3, 1, n, 3, m, 40, ds, out, opts)
+    D.synchronize()
+    D.device_status()
+    want = O.npdu(q, m, 40, 77)
+    np.testing.assert_array_equal(out.numpy(), want, err_msg=plan)
+    np.testing.assert_array_equal(opts.numpy(), q[want.astype(np.int64)], err_msg=plan)

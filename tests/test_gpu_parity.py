@@ -387,3 +387,26 @@ def test_lattice_streaming_kernels_agree(gpu, plan_env):
     b = run_fps(gpu, pc, 4096, 0)
     np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(a[:24], O.fps(pc, 24, 0))
+
+
+@pytest.mark.parametrize("plan", ["pruned:1024:4", "pair:1024:4", "spruned:3"])
+def test_fused_gather_pruned_kinds(gpu, plan_env, plan):
+    # The pruned, paired (both picks of a round) and streaming pruned kernels
+    # write the picks' coordinates from their own pick records; two starts,
+    # an odd sample count, SoA device data.
+    from fpsample_amd import device as D
+
+    plan_env(plan)
+    n, m = 20000, 2001
+    pc = c2_cloud(n, seed=9)
+    want = O.fps(pc, m, [17, 3])
+    soa = D.DeviceArray.from_numpy(np.ascontiguousarray(pc.T))
+    ds = D.DeviceArray.from_numpy(np.array([17, 3], np.uint64))
+    out = D.DeviceArray((m,), np.uint64)
+    opts = D.DeviceArray((m, 3), np.float32)
+    log(f"fused gather {plan}")
+    D.fps_sampling_device_strided(soa, 1, n, n, 3, m, ds, 2, out, opts)
+    D.synchronize()
+    D.device_status()
+    np.testing.assert_array_equal(out.numpy(), want, err_msg=plan)
+    np.testing.assert_array_equal(opts.numpy(), pc[want.astype(np.int64)], err_msg=plan)
