@@ -184,8 +184,9 @@ static int plan_stream(Ctx &c, size_t n, size_t dim, int tie, int B, size_t want
   return FPSX_OK;
 }
 
-// Streaming pruned plan: 1024-thread workgroups; G chosen so that a
-// workgroup caches about 256 blocks (LDS metadata ~11 KB), bounded by
+// Streaming pruned plan: 1024-thread workgroups; G = 64 (lattice16m at 32,
+// 64, 128 and 256 workgroups: 23.2, 20.3, 22.2 and 26.2 ms), more where a
+// workgroup would cache over 2048 blocks (96 KB of LDS metadata), bounded by
 // co-residency and the exchange limits.  want_g > 0 forces G.
 static int plan_stream_pruned(Ctx &c, size_t n, int tie, size_t want_g, Plan &p) {
   size_t cnt;
@@ -195,7 +196,8 @@ static int plan_stream_pruned(Ctx &c, size_t n, int tie, size_t want_g, Plan &p)
     if (t[i].tie == tie) k = &t[i];
   if (!k) return fail(FPSX_EINVAL, "no streaming pruned kernel");
   const size_t nblocks = (n + kSBlock - 1) / kSBlock;
-  size_t g = want_g ? want_g : std::max<size_t>(1, (nblocks + 255) / 256);
+  size_t g = want_g ? want_g
+                    : std::max<size_t>(std::min<size_t>(64, nblocks), (nblocks + 2047) / 2048);
   g = std::min<size_t>(g, std::min<size_t>(nblocks, 256));
   size_t nbl = (nblocks + g - 1) / g;
   size_t shmem = (size_t)kSMetaWords * nbl * sizeof(int);
