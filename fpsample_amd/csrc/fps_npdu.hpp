@@ -20,9 +20,11 @@ const KernelEntry *npdu_table(size_t *count);
 const KernelEntry *npdu_pair_table(size_t *count);
 constexpr uint32_t kNpduPairMaxGrid = 64;
 
-// Streaming form (any dimension, ppt 0): blocks of kNpduBlock indices; LDS
-// holds dim floats of pivot plus 2 words per block of the workgroup.
-constexpr int kNpduBlock = 1024;
+// Streaming forms (ppt 0): blocks of kNpduBlock indices; LDS holds 2 words
+// per block of the workgroup plus dim floats of pivot (any dimension), or 5
+// words per block (3-D).  256 indices (4 M 3-D points, 4096 samples, 64
+// workgroups: blocks of 1024, 512 and 256 take 19.3, 15.0 and 13.5 ms).
+constexpr int kNpduBlock = 256;
 // Streaming windowed-update kernel: any dimension, or the 3-D form that
 // carries the winner's coordinates through the exchange.
 const KernelEntry *npdu_stream_kernel(bool three_d);

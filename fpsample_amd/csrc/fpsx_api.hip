@@ -383,15 +383,18 @@ static const KernelEntry *find_npdu(int dim, int B, int ppt) {
 // suffices (16384 / 32768 points); beyond, 1024-thread workgroups with the
 // share per thread that minimises 0.012 us per slot + 0.0125 us per
 // workgroup of exchange (fitted to MI355X sweeps at 50 K and 100 K points).
-// Streaming windowed-update plan: G chosen so that a workgroup caches about
-// 256 blocks; want_g > 0 forces G.  3-D clouds take the kernel that carries
-// the winner's coordinates (5 LDS words per block) unless any_dim is set;
-// the any-dimension kernel keeps the pivot (dim words) and 2 words per block.
+// Streaming windowed-update plan: G = 64 (4 M and 9 M points at 32, 64 and
+// 128 workgroups: 14.1 / 13.5 / 16.8 and 18.4 / 15.3 / 19.0 ms), more where a
+// workgroup would cache over 4096 blocks; want_g > 0 forces G.  3-D clouds
+// take the kernel that carries the winner's coordinates (5 LDS words per
+// block) unless any_dim is set; the any-dimension kernel keeps the pivot
+// (dim words) and 2 words per block.
 static int plan_npdu_stream(Ctx &c, size_t n, size_t dim, size_t want_g, Plan &p,
                             bool any_dim = false) {
   const KernelEntry *k = npdu_stream_kernel(dim == 3 && !any_dim);
   const size_t nblocks = (n + kNpduBlock - 1) / kNpduBlock;
-  size_t g = want_g ? want_g : std::max<size_t>(1, (nblocks + 255) / 256);
+  size_t g = want_g ? want_g
+                    : std::max<size_t>(std::min<size_t>(64, nblocks), (nblocks + 4095) / 4096);
   g = std::min<size_t>(g, std::min<size_t>(nblocks, 256));
   const size_t nbl = (nblocks + g - 1) / g;
   const size_t shmem = (k->dim == 3) ? 5 * nbl * sizeof(int)
