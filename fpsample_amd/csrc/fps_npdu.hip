@@ -592,18 +592,20 @@ __global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
 // ---------------------------------------------------------------------------
 // Streaming form for clouds beyond the register file, any dimension.  The
 // min-distances live in HBM (f.dmin, by index) and the cloud is cut into
-// blocks of kNpduBlock consecutive indices; block b belongs to workgroup
+// blocks of BLK consecutive indices (kNpduBlock, or kNpduBigBlock where a
+// workgroup's cache would not fit LDS); block b belongs to workgroup
 // b mod G, which caches in LDS each of its blocks' best (value, index).  A
 // pass re-reads only the blocks the window reaches (every block for the
 // start): in-window min-distances are updated, and the block's best is
 // re-taken over all its min-distances.  The winner travels as (value,
 // index); every workgroup then re-reads the pivot's coordinates.
 // ---------------------------------------------------------------------------
+template <int BLK>
 __global__ void __launch_bounds__(1024) fps_npdu_stream(NpduParams a) {
   constexpr int B = 1024;
   constexpr int W = B / 64;
   constexpr int NW = 2;
-  constexpr int SL = kNpduBlock / 64;
+  constexpr int SL = BLK / 64;
   __shared__ int s_cand[2][NW][W];
   __shared__ int s_xw[2][NW + 1][W];
   extern __shared__ int s_dyn[];  // pivot coordinates (dim), block values (nbl), block indices (nbl)
@@ -616,7 +618,7 @@ __global__ void __launch_bounds__(1024) fps_npdu_stream(NpduParams a) {
   const uint32_t g = blockIdx.x;
   const uint32_t n = f.n;
   const uint32_t dim = f.dim;
-  const uint32_t nblocks = (n + kNpduBlock - 1) / kNpduBlock;
+  const uint32_t nblocks = (n + BLK - 1) / BLK;
   const uint32_t nbl = (nblocks > g) ? (nblocks - g + G - 1) / G : 0u;
   float *s_pv = reinterpret_cast<float *>(s_dyn);
   int *m_v = s_dyn + dim;
@@ -659,11 +661,11 @@ __global__ void __launch_bounds__(1024) fps_npdu_stream(NpduParams a) {
       hi = (uint32_t)h;
     }
     // Local slots j whose block g + j*G meets [lo, hi].
-    const uint32_t blo = lo / kNpduBlock, bhi = hi / kNpduBlock;
+    const uint32_t blo = lo / BLK, bhi = hi / BLK;
     const uint32_t j_first = (blo > g) ? (blo - g + G - 1) / G : 0u;
     const uint32_t j_end = (bhi >= g) ? min(nbl, (bhi - g) / G + 1) : 0u;
     for (uint32_t j = j_first + (uint32_t)wave; j < j_end; j += W) {
-      const uint32_t base = (g + j * G) * (uint32_t)kNpduBlock;
+      const uint32_t base = (g + j * G) * (uint32_t)BLK;
       // Every load of the block before any update or store: a store between
       // two slots' loads would hold the later loads behind it (the compiler
       // cannot rule out aliasing), one memory round trip per slot and
@@ -939,11 +941,13 @@ __global__ void __launch_bounds__(1024) fps_npdu_stream3(NpduParams a) {
 }
 
 static const KernelEntry kNpduStream[] = {
-    {0, 1024, 0, kTieLowest, (const void *)&fps_npdu_stream},
+    {0, 1024, 0, kTieLowest, (const void *)&fps_npdu_stream<kNpduBlock>},
     {3, 1024, 0, kTieLowest, (const void *)&fps_npdu_stream3},
+    {0, 1024, 0, kTieLowest, (const void *)&fps_npdu_stream<kNpduBigBlock>},
 };
 
 const KernelEntry *npdu_stream_kernel(bool three_d) { return &kNpduStream[three_d ? 1 : 0]; }
+const KernelEntry *npdu_stream_kernel_big() { return &kNpduStream[2]; }
 
 #define FPSX_NPDU(D, B, P) {D, B, P, kTieLowest, (const void *)&fps_npdu<D, B, P>}
 

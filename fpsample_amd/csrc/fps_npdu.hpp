@@ -26,7 +26,11 @@ constexpr uint32_t kNpduPairMaxGrid = 64;
 // workgroups: blocks of 1024, 512 and 256 take 19.3, 15.0 and 13.5 ms).
 constexpr int kNpduBlock = 256;
 // Streaming windowed-update kernel: any dimension, or the 3-D form that
-// carries the winner's coordinates through the exchange.
+// carries the winner's coordinates through the exchange (both with blocks of
+// kNpduBlock indices); the any-dimension kernel with blocks of kNpduBigBlock
+// for clouds whose per-workgroup cache would not fit LDS otherwise.
+constexpr int kNpduBigBlock = 1024;
 const KernelEntry *npdu_stream_kernel(bool three_d);
+const KernelEntry *npdu_stream_kernel_big();
 
 }  // namespace fpsx

@@ -390,27 +390,43 @@ static const KernelEntry *find_npdu(int dim, int B, int ppt) {
 // block) unless any_dim is set; the any-dimension kernel keeps the pivot
 // (dim words) and 2 words per block.
 static int plan_npdu_stream(Ctx &c, size_t n, size_t dim, size_t want_g, Plan &p,
-                            bool any_dim = false) {
-  const KernelEntry *k = npdu_stream_kernel(dim == 3 && !any_dim);
-  const size_t nblocks = (n + kNpduBlock - 1) / kNpduBlock;
-  size_t g = want_g ? want_g
-                    : std::max<size_t>(std::min<size_t>(64, nblocks), (nblocks + 4095) / 4096);
-  g = std::min<size_t>(g, std::min<size_t>(nblocks, 256));
-  const size_t nbl = (nblocks + g - 1) / g;
-  const size_t shmem = (k->dim == 3) ? 5 * nbl * sizeof(int)
-                                     : dim * sizeof(float) + 2 * nbl * sizeof(int);
-  if (shmem > 96 * 1024)
-    return fail(FPSX_ETOOBIG, "windowed-update streaming plan: too many blocks per workgroup");
-  const size_t gmax = (size_t)c.max_blocks(k, shmem);
-  if (g > gmax)
-    return fail(FPSX_ETOOBIG, "windowed-update streaming plan: grid exceeds co-residency");
-  p.kind = PlanKind::Npdu;
-  p.k = k;
-  p.block = 1024;
-  p.grid = (uint32_t)g;
-  p.chunk = (uint32_t)(nbl * kNpduBlock);
-  p.shmem = shmem;
-  return FPSX_OK;
+                            bool any_dim = false, bool big_only = false) {
+  // Candidates in order: the 3-D kernel, the any-dimension kernel, the
+  // any-dimension kernel with 4x larger blocks (clouds whose per-workgroup
+  // cache of small blocks would not fit 96 KB of LDS: beyond about 322 M 3-D
+  // points, or 805 M with the any-dimension kernel).
+  struct Cand {
+    const KernelEntry *k;
+    size_t blk;
+  };
+  Cand cands[3];
+  int nc = 0;
+  if (dim == 3 && !any_dim && !big_only) cands[nc++] = {npdu_stream_kernel(true), (size_t)kNpduBlock};
+  if (!big_only) cands[nc++] = {npdu_stream_kernel(false), (size_t)kNpduBlock};
+  cands[nc++] = {npdu_stream_kernel_big(), (size_t)kNpduBigBlock};
+  for (int i = 0; i < nc; ++i) {
+    const KernelEntry *k = cands[i].k;
+    const size_t blk = cands[i].blk;
+    const size_t nblocks = (n + blk - 1) / blk;
+    size_t g = want_g ? want_g
+                      : std::max<size_t>(std::min<size_t>(64, nblocks), (nblocks + 4095) / 4096);
+    g = std::min<size_t>(g, std::min<size_t>(nblocks, 256));
+    const size_t nbl = (nblocks + g - 1) / g;
+    const size_t shmem = (k->dim == 3) ? 5 * nbl * sizeof(int)
+                                       : dim * sizeof(float) + 2 * nbl * sizeof(int);
+    if (shmem > 96 * 1024) continue;
+    const size_t gmax = (size_t)c.max_blocks(k, shmem);
+    if (g > gmax)
+      return fail(FPSX_ETOOBIG, "windowed-update streaming plan: grid exceeds co-residency");
+    p.kind = PlanKind::Npdu;
+    p.k = k;
+    p.block = 1024;
+    p.grid = (uint32_t)g;
+    p.chunk = (uint32_t)(nbl * blk);
+    p.shmem = shmem;
+    return FPSX_OK;
+  }
+  return fail(FPSX_ETOOBIG, "windowed-update streaming plan: too many blocks per workgroup");
 }
 
 static const KernelEntry *find_npdu_pair(int B, int ppt) {
@@ -424,11 +440,13 @@ static const KernelEntry *find_npdu_pair(int B, int ppt) {
 int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p) {
   const char *env = std::getenv("FPSX_NPDU_PLAN");
   if (env && std::strncmp(env, "stream", 6) == 0) {
-    // "stream[:G]"; "stream0[:G]" forces the any-dimension kernel.
+    // "stream[:G]"; "stream0[:G]" forces the any-dimension kernel,
+    // "streambig[:G]" its large-block instance.
     const bool any = std::strncmp(env, "stream0", 7) == 0;
+    const bool big = std::strncmp(env, "streambig", 9) == 0;
     int g = 0;
-    const int got = std::sscanf(env + (any ? 7 : 6), ":%d", &g);
-    return plan_npdu_stream(c, n, dim, got == 1 ? (size_t)g : 0, p, any);
+    const int got = std::sscanf(env + (big ? 9 : any ? 7 : 6), ":%d", &g);
+    return plan_npdu_stream(c, n, dim, got == 1 ? (size_t)g : 0, p, any || big, big);
   }
   if (env && std::strncmp(env, "pair:", 5) == 0) {
     int b = 0, q = 0;

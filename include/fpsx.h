@@ -180,7 +180,9 @@ int fpsx_fps_npdu_sampling_device(const float *d_points, size_t stride_point, si
  * streaming form (min-distances in HBM, used automatically beyond the
  * register file; ppt 0, kernel_dim 3 for 3-D clouds, whose winners carry
  * their coordinates through the exchange, else 0), "stream0[:G]" its
- * any-dimension kernel on any cloud.
+ * any-dimension kernel on any cloud, "streambig[:G]" that kernel with 4x
+ * larger blocks (taken automatically where the per-workgroup cache of small
+ * blocks would not fit LDS).
  */
 int fpsx_npdu_plan_info(size_t n_points, size_t dim, int *kernel_dim, int *block, int *ppt,
                         int *grid);

@@ -201,11 +201,12 @@ def test_npdu_error_codes(gpu):
 
 
 @pytest.mark.parametrize("plan", ["stream", "stream:1", "stream:4", "stream:13", "stream0",
-                                  "stream0:4"])
+                                  "stream0:4", "streambig", "streambig:3"])
 def test_npdu_streaming_form(gpu, npdu_plan, plan):
     # Min-distances in HBM, per-block cached bests, window blocks only.  3-D
     # clouds take the kernel whose winners carry their coordinates; "stream0"
-    # forces the any-dimension kernel on them too.
+    # forces the any-dimension kernel on them too, "streambig" its
+    # large-block instance (the planner's choice beyond about 805 M points).
     npdu_plan(plan)
     rng = np.random.default_rng(47)
     for name, pc, w in (("uniform3", rng.random((20000, 3)).astype(np.float32), 40),
