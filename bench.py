@@ -4,32 +4,37 @@
 A step is one vanilla FPS run (fps_sampling semantics, src/lib.cpp:188-246)
 over one synthetic cloud already resident in HBM: one call of
 fpsx_fps_sampling_device, i.e. the exchange-area clear, for large 3-D clouds
-the spatial ordering of the points (bounding box, Morton codes, two radix
+the spatial ordering of the points (bounding box, Hilbert codes, two radix
 sorts, gather), and the single persistent FPS launch that makes all M-1
 passes.  The kernel shape comes from the library's planner; --autotune on
-times every shape first (untimed) instead.  The npdu* workloads time
+times every plan first (untimed) instead.  The npdu* workloads time
 fps_npdu_sampling (src/lib.cpp:272-340) the same way, through
 fpsx_fps_npdu_sampling_device on row-major device data.
 
 Default workload (the metric's configuration): one cloud of N = 1,048,576
 points, D = 3 float32, a mixture of 64 isotropic Gaussian clusters (sigma
 0.02, centres uniform in [0,1)^3, numpy default_rng(1)), M = 16,384 samples,
-start_idx = 0.
+start_idx = 0.  The same JSON line then carries the metric's second half:
+  serve     the serving workload (96 requests, N log-uniform in [2^10, 2^20],
+            M = N * U{1/8,1/4,1/2}): capacity as one burst, p50/p99 latency
+            under Poisson arrivals at 60 % of it, and the same requests run
+            one launch per request, with outputs compared;
+  batch256  the 256 ragged clouds in one persistent launch against one
+            launch per cloud, outputs compared.
 
 Prints one JSON line.  value = samples/s of the whole job (M x steps / wall
-time between two device synchronisations).  roofline: algorithmic bytes of
-the vanilla loop, 20 B per point-pass (x, y, z and min-distance read, the
-min-distance written) x N x (M - 1) per launch, divided by the FPS kernel's
-average duration measured with HIP events on the stream it runs on, against
-the 8 TB/s HBM peak.  cpu_baseline: the reference's fps_sampling compiled
-from its own sources (oracle/_ref) -- or the oracle's C restatement where
-that build is absent -- timed single-threaded on this host over a bounded
-number of samples of the same cloud (its per-sample cost does not depend on
-how many samples are drawn).  The same CPU leg reports accuracy_vs_fp64
-(index agreement and coverage-radius relative error of the GPU picks against
-the greedy loop in double precision) and, for the 1 M-point cloud,
-bucket_kdline: the bucket entry (h = 9) on the GPU beside the reference's
-bucket sampler on one host core, with their picks compared.
+time between two device synchronisations).  roofline: what bounds the
+kernel.  Multi-workgroup resident and pruned kernels are bound by the
+latency of their per-pass cross-workgroup exchange, not by HBM: the line
+reports exchanges per second against what the exchange alone sustains at
+the same grid and candidate size (fpsx_probe_exchange, measured in the same
+run), plus the measured HBM traffic and LDS bytes (committed rocprofv3
+counter summaries) and, as vs_vanilla_hbm, the vanilla loop's 20 B per
+point-pass over the kernel time.  cpu_baseline: the reference's
+fps_sampling compiled from its own sources (oracle/_ref; the oracle's C
+restatement where that build is absent), single-threaded and pinned to one
+core of this host, over all M samples of the same cloud when they fit the
+time budget.
 """
 from __future__ import annotations
 
@@ -131,43 +136,86 @@ WORKLOADS = {
 
 
 # ---------------------------------------------------------------- CPU baseline
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+class _pinned:
+    """Run the enclosed CPU timing on one core (the first this process may use)."""
+
+    def __enter__(self):
+        self.saved = None
+        if hasattr(os, "sched_setaffinity"):
+            self.saved = os.sched_getaffinity(0)
+            self.core = min(self.saved)
+            os.sched_setaffinity(0, {self.core})
+        else:
+            self.core = None
+        return self
+
+    def __exit__(self, *exc):
+        if self.saved is not None:
+            os.sched_setaffinity(0, self.saved)
+
+
 def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float, w: int = -1) -> dict:
-    """w < 0: fps_sampling; else fps_npdu_sampling with window w."""
+    """w < 0: fps_sampling; else fps_npdu_sampling with window w.  The
+    reference (oracle/_ref) single-threaded on one pinned core, over all m_full
+    samples when the calibrated estimate fits target_s (else the largest
+    prefix that does, stated in `sample`); the oracle's C restatement is timed
+    beside it on a shorter prefix (port_value)."""
     from oracle import oracle as O
 
     ref = O.load_reference()
-    if ref is not None:
-        kind = "reference"
+
+    def runner(which):
+        if which == "reference":
+            if w < 0:
+                return lambda m: ref.fps_sampling(pc, m, 0)
+            return lambda m: ref.fps_npdu(pc, m, w, 0)
         if w < 0:
-            run = lambda m: ref.fps_sampling(pc, m, 0)  # noqa: E731
-        else:
-            run = lambda m: ref.fps_npdu(pc, m, w, 0)  # noqa: E731
-    else:
-        kind = "port"
-        if w < 0:
-            run = lambda m: O.fps(pc, m, 0)  # noqa: E731
-        else:
-            run = lambda m: O.npdu(pc, m, w, 0)  # noqa: E731
+            return lambda m: O.fps(pc, m, 0)
+        return lambda m: O.npdu(pc, m, w, 0)
+
+    kind = "reference" if ref is not None else "port"
+    run = runner(kind)
     n = pc.shape[0]
-    # calibrate on a few passes, then size the sample to about target_s seconds
-    m_cal = max(2, min(m_full, int(2e8 // max(n, 1)) + 2))
-    t0 = time.perf_counter()
-    run(m_cal)
-    dt = time.perf_counter() - t0
-    per_pass = dt / max(m_cal - 1, 1)
-    m_s = int(min(m_full, max(m_cal, target_s / max(per_pass, 1e-12))))
-    t0 = time.perf_counter()
-    run(m_s)
-    dt = time.perf_counter() - t0
-    return {
+    with _pinned() as pin:
+        # calibrate on a few passes, then size the sample to the time budget
+        m_cal = max(2, min(m_full, int(2e8 // max(n, 1)) + 2))
+        t0 = time.perf_counter()
+        run(m_cal)
+        per_pass = (time.perf_counter() - t0) / max(m_cal - 1, 1)
+        m_s = int(min(m_full, max(m_cal, target_s / max(per_pass, 1e-12))))
+        t0 = time.perf_counter()
+        run(m_s)
+        dt = time.perf_counter() - t0
+        port = None
+        if kind == "reference":
+            m_p = max(2, min(m_s, m_cal * 4))
+            t0 = time.perf_counter()
+            runner("port")(m_p)
+            port = m_p / (time.perf_counter() - t0)
+    what = (f"all {m_s} samples" if m_s == m_full
+            else f"the first {m_s} of the {m_full} samples ({100.0 * m_s / m_full:.1f} %)")
+    out = {
         "value": round(m_s / dt, 3),
         "unit": "samples/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"first {m_s} of the {m_full} samples of the same cloud, start_idx=0, "
-                  f"{dt:.2f} s single-threaded on {platform.processor() or platform.machine()} "
-                  f"(os.cpu_count()={os.cpu_count()})",
+        "sample": f"{what} of the same cloud, start_idx=0, {dt:.2f} s single-threaded on core "
+                  f"{pin.core} of a {cpu_model()} (os.cpu_count()={os.cpu_count()})",
     }
+    if port is not None:
+        out["port_value"] = round(port, 3)
+        out["port_over_reference"] = round(port / out["value"], 3)
+    return out
 
 
 def accuracy_and_bucket(pc: np.ndarray, gpu_picks: np.ndarray, m_full: int, prefix: int = 1024,
@@ -210,11 +258,13 @@ def accuracy_and_bucket(pc: np.ndarray, gpu_picks: np.ndarray, m_full: int, pref
     return out
 
 
-def _read_traffic(workload: str, plan: dict):
-    """HBM bytes per launch of the FPS kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known
-    size), when that summary was taken with the plan this run uses."""
-    path = os.path.join(ROOT, "profiles", f"fps_{plan.get('kind')}_{workload}_traffic.txt")
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B per clock x 2.4 GHz
+
+
+def _read_summary(workload: str, plan: dict, what: str):
+    """key=value lines of profiles/fps_<kind>_<workload>_<what>.txt, when that
+    rocprofv3 summary was taken with the plan this run uses (else None)."""
+    path = os.path.join(ROOT, "profiles", f"fps_{plan.get('kind')}_{workload}_{what}.txt")
     if not os.path.exists(path):
         return None
     kv = {}
@@ -223,12 +273,122 @@ def _read_traffic(workload: str, plan: dict):
             k, v = line.strip().split("=", 1)
             kv[k] = v
     want = f"{plan.get('kind')}:{plan.get('block')}:{plan.get('ppt')}:{plan.get('grid')}"
-    if kv.get("plan") != want:
-        return None
+    return kv if kv.get("plan") == want else None
+
+
+def _float(kv, key):
     try:
-        return float(kv["traffic_bytes_per_launch"])
+        return float(kv[key]) if kv is not None else None
     except (KeyError, ValueError):
         return None
+
+
+def probe_exchange(grid: int, words: int, rounds: int = 20000) -> float:
+    """Microseconds per round of the exchange alone (fpsx_probe_exchange)."""
+    import ctypes
+
+    from fpsample_amd import device as D
+
+    L = D.lib()
+    L.fpsx_probe_exchange.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_double)]
+    us = ctypes.c_double()
+    best = None
+    for _ in range(3):  # the fastest of three launches (the first also warms up)
+        D._raise(L.fpsx_probe_exchange(grid, words, rounds, None, ctypes.byref(us)),
+                 "fpsx_probe_exchange")
+        best = us.value if best is None else min(best, us.value)
+    return best
+
+
+def launch_exchanges() -> int:
+    import ctypes
+
+    from fpsample_amd import device as D
+
+    L = D.lib()
+    L.fpsx_launch_exchanges.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+    x = ctypes.c_uint64()
+    D._raise(L.fpsx_launch_exchanges(None, ctypes.byref(x)), "fpsx_launch_exchanges")
+    return int(x.value)
+
+
+def roofline(workload: str, plan: dict, n: int, m: int, kernel_s: float, npdu: bool,
+             w: int = -1) -> dict:
+    """What bounds the FPS kernel, measured in this run.
+
+    Streaming kernels (plain or generic) read and write every point's state
+    every pass: bound "hbm", the vanilla loop's 20 B per point-pass over the
+    kernel time against the 8 TB/s peak.  Multi-workgroup resident and pruned
+    kernels keep the state on chip: bound "latency", the exchanges the
+    launch made per second against the rate the exchange alone sustains at
+    the same grid and candidate size (frac = exchange floor / kernel time,
+    at most 1).  The measured HBM traffic and LDS bytes come from the
+    committed rocprofv3 summaries for this plan."""
+    if npdu:
+        alg = (20 * (min(w, n - 1) + 1) + 4 * n) * (m - 1)
+    else:
+        alg = BYTES_PER_POINT_PASS * n * (m - 1)
+    vanilla = {"algorithmic_bytes_per_launch": alg,
+               "achieved": round(alg / kernel_s / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(alg / kernel_s / 1e9 / HBM_PEAK_GBS, 4)}
+    traffic = _float(_read_summary(workload, plan, "traffic"), "traffic_bytes_per_launch")
+    counters = _read_summary(workload, plan, "counters")
+    lds_bytes = _float(counters, "lds_bytes_per_launch")
+    out = {"kernel_ms": round(kernel_s * 1e3, 4), "traffic": traffic}
+    if traffic is not None:
+        out["hbm"] = {"achieved": round(traffic / kernel_s / 1e9, 2), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(traffic / kernel_s / 1e9 / HBM_PEAK_GBS, 5)}
+    if lds_bytes is not None:
+        out["lds"] = {"bytes_per_launch": lds_bytes,
+                      "achieved": round(lds_bytes / kernel_s / 1e9, 2), "peak": LDS_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(lds_bytes / kernel_s / 1e9 / LDS_PEAK_GBS, 5),
+                      "bank_conflict_cycles": _float(counters, "SQ_LDS_BANK_CONFLICT")}
+    kind, grid = plan.get("kind"), int(plan.get("grid", 1))
+    if kind == "stream" and not npdu:
+        out.update(bound="hbm", achieved=vanilla["achieved"], peak=HBM_PEAK_GBS, unit="GB/s",
+                   frac=vanilla["frac"], algorithmic_bytes_per_launch=alg)
+        return out
+    out["vs_vanilla_hbm"] = vanilla
+    if grid < 2:
+        # One workgroup: no exchange; the pass is the update and the
+        # workgroup's reduction chain.
+        out.update(bound="latency", achieved=None, peak=None, unit="exchanges/s", frac=None,
+                   note="one workgroup, no cross-workgroup exchange")
+        return out
+    if npdu:
+        exchanges = None
+        words = 5
+    else:
+        exchanges = launch_exchanges()
+        words = 10 if kind == "pair" else 5
+    floor_us = probe_exchange(grid, words)
+    if exchanges is None:
+        out.update(bound="latency", achieved=None, peak=round(1e6 / floor_us, 1),
+                   unit="exchanges/s", frac=None, exchange_floor_us=round(floor_us, 4))
+        return out
+    achieved = exchanges / kernel_s
+    out.update(bound="latency", achieved=round(achieved, 1), peak=round(1e6 / floor_us, 1),
+               unit="exchanges/s", frac=round(achieved * floor_us / 1e6, 4),
+               exchanges_per_launch=exchanges, exchange_floor_us=round(floor_us, 4),
+               us_per_exchange=round(kernel_s * 1e6 / exchanges, 4),
+               floor_probe={"grid": grid, "words": words, "threads": 1024})
+    return out
+
+
+def _reference_check(name: str, picks_list) -> object:
+    """True/False: every pick sequence's SHA-256 equals the one the reference
+    wrote into the full-length fixture `name`; None where that fixture is absent."""
+    try:
+        from tests.golden import full_length as FL
+    except ImportError:
+        return None
+    if not FL.available(name):
+        return None
+    d = FL.load(name)
+    if len(d["picks_sha256"]) != len(picks_list):
+        return False
+    return all(FL.picks_sha256(p) == str(h) for p, h in zip(picks_list, d["picks_sha256"]))
 
 
 def run_batch(args) -> dict:
@@ -271,77 +431,114 @@ def run_batch(args) -> dict:
     D.synchronize()
     t1 = time.perf_counter()
     wall_p = (t1 - t0) / reps
-    same = all(np.array_equal(a, o.numpy()) for a, o in zip(batched, outs))
-    alg = sum(BYTES_PER_POINT_PASS * pc.shape[0] * (m - 1) for pc, m in zip(clouds, ms))
-    achieved = alg / wall_b / 1e9
+    single = [o.numpy() for o in outs]
+    same = all(np.array_equal(a, o) for a, o in zip(batched, single))
+    b.close()
     return {
         "metric": "samples/s (256 ragged clouds, one persistent launch)",
         "value": round(total_m / wall_b, 3),
         "unit": "samples/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(wall_b * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
+        "ms_per_step": round(wall_b * 1e3, 4), "steps": args.steps,
         "config": {"workload": "B=256 clouds, N_i log-uniform [1024, 65536], M_i = N_i/4, D=3 f32, "
-                               "half uniform cube / half unit sphere + N(0,0.01^2), start_idx=0",
+                               "half uniform cube / half unit sphere + N(0,0.01^2), start_idx=0, "
+                               "HIP graph, autotuned shape",
                    "total_points": int(sum(pc.shape[0] for pc in clouds)), "total_samples": total_m,
                    "batch_kernel": info},
         "one_cloud_per_launch": {"value": round(total_m / wall_p, 3), "unit": "samples/s",
                                  "ms_per_step": round(wall_p * 1e3, 3)},
         "speedup_vs_one_cloud_per_launch": round(wall_p / wall_b, 3),
+        "target_speedup": 5.0,
         "identical_output": bool(same),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "identical_to_reference": _reference_check("batch256", batched),
     }
 
 
 def run_serve(args) -> dict:
-    """Serving stream: Poisson arrivals at ~60% of the measured capacity."""
+    """Serving workload: capacity (every request at once), latency under
+    Poisson arrivals at 60 % of that capacity, and the same requests one
+    launch per request on device-resident clouds."""
+    from fpsample_amd import device as D
     from fpsample_amd.serve import Server
 
     reqs = gen_serving_stream()
+    total_m = sum(m for _, m in reqs)
     with Server(hbm_budget_bytes=16 << 30, chunk_passes=256) as srv:
-        # capacity: the first third of the requests submitted as one burst
-        cal = reqs[: len(reqs) // 3]
+        srv.wait(srv.submit(reqs[0][0], reqs[0][1], 0))  # warm-up
         t0 = time.perf_counter()
-        ids = [srv.submit(pc, m, 0) for pc, m in cal]
-        for r in ids:
-            srv.wait(r)
-        cap = len(cal) / (time.perf_counter() - t0)
+        ids = [srv.submit(pc, m, 0) for pc, m in reqs]
+        burst = [srv.wait(r) for r in ids]
+        t_burst = time.perf_counter() - t0
+        cap = len(reqs) / t_burst
         qps = 0.6 * cap
-        rng = np.random.default_rng(5)
-        stream = reqs[len(reqs) // 3:]
-        gaps = rng.exponential(1.0 / qps, len(stream))
+        gaps = np.random.default_rng(5).exponential(1.0 / qps, len(reqs))
         ids = []
         t0 = time.perf_counter()
         t_next = t0
-        for (pc, m), g in zip(stream, gaps):
+        for (pc, m), g in zip(reqs, gaps):
             t_next += g
             now = time.perf_counter()
             if t_next > now:
                 time.sleep(t_next - now)
             ids.append(srv.submit(pc, m, 0))
-        lats = [srv.wait(r)[1] for r in ids]
-        t1 = time.perf_counter()
+        streamed = [srv.wait(r) for r in ids]
+        t_stream = time.perf_counter() - t0
         st = srv.stats()
-    lat = np.array(lats) * 1e3
-    samples = sum(m for _, m in stream)
+    lat = np.array([x[1] for x in streamed]) * 1e3
+    served = [x[0] for x in burst]
+    same_stream = all(np.array_equal(a[0], b) for a, b in zip(streamed, served))
+    # one request per launch: clouds resident first (no upload timed), the
+    # planner's plan per request, launches back to back on one stream
+    dev = [D.DeviceArray.from_numpy(np.ascontiguousarray(pc.T)) for pc, _ in reqs]
+    outs = [D.DeviceArray((m,), np.uint64) for _, m in reqs]
+    d_start = D.DeviceArray.from_numpy(np.zeros(1, np.uint64))
+    D.synchronize()
+    t0 = time.perf_counter()
+    for (pc, m), dp, do in zip(reqs, dev, outs):
+        D.fps_sampling_device(dp, pc.shape[0], 3, m, d_start, 1, do)
+    D.synchronize()
+    t_single = time.perf_counter() - t0
+    D.device_status()
+    single = [o.numpy() for o in outs]
+    same = all(np.array_equal(a, b) for a, b in zip(served, single))
     return {
         "metric": "p50/p99 latency, mixed FPS stream",
-        "value": round(samples / (t1 - t0), 3),
-        "unit": "samples/s",
-        "n_gpus": 1, "steps": len(stream), "warmup": len(cal),
-        "ms_per_step": round((t1 - t0) / len(stream) * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "Poisson arrivals, N log-uniform [2^10, 2^20], uniform cube, "
-                               "M = N*U{1/8,1/4,1/2}, start_idx=0, HBM budget 16 GiB, 256-pass chunks",
-                   "qps": round(qps, 3), "measured_capacity_rps": round(cap, 3)},
+        "config": {"workload": "96 requests, N log-uniform [2^10, 2^20], uniform cube, "
+                               "M = N*U{1/8,1/4,1/2}, start_idx=0, HBM budget 16 GiB, "
+                               "256-pass chunks, batch kernel 1024 x 16",
+                   "total_samples": total_m, "qps": round(qps, 3)},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "requests_per_s": round(len(stream) / (t1 - t0), 3),
+        "stream_requests_per_s": round(len(reqs) / t_stream, 3),
+        "stream_samples_per_s": round(total_m / t_stream, 3),
+        "capacity": {"requests_per_s": round(cap, 3), "samples_per_s": round(total_m / t_burst, 3),
+                     "s": round(t_burst, 3)},
+        "one_request_per_launch": {"samples_per_s": round(total_m / t_single, 3),
+                                   "s": round(t_single, 3),
+                                   "note": "clouds already in HBM, uploads not timed"},
+        "speedup_vs_one_request_per_launch": round(t_single / t_burst, 3),
+        "target_speedup": 5.0,
+        "identical_output": bool(same and same_stream),
+        "identical_to_reference": _reference_check("serve96", served),
         "server": st,
     }
+
+
+def run_pingpong(args) -> dict:
+    """The exchange alone (fpsx_probe_exchange) over grid sizes and candidate
+    sizes: the per-pass latency floor of the multi-workgroup kernels."""
+    rows = []
+    for words in (5, 10):
+        for g in (2, 8, 16, 32, 64, 128, 256):
+            if words == 10 and g > 64:
+                continue
+            rows.append({"grid": g, "words": words, "us_per_round": round(probe_exchange(g, words), 4)})
+    head = next(r for r in rows if r["grid"] == 64 and r["words"] == 10)
+    return {"metric": "exchange-only round time, 64 workgroups, 10-word candidates (paired kernel)",
+            "value": head["us_per_round"], "unit": "us", "n_gpus": 1, "steps": 1, "warmup": 2,
+            "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u64", "data": "synthetic",
+            "config": {"workload": "exchange-only probe, 1024-thread workgroups, 20000 rounds"},
+            "rows": rows}
 
 
 def run_copy(args) -> dict:
@@ -371,17 +568,22 @@ def run_copy(args) -> dict:
             "config": {"workload": "hipMemcpy device-to-device, 268435456 bytes per copy"}}
 
 
+METRIC = "samples/s at N=1M,M=16K (+% roofline); p50/p99 latency, mixed FPS stream"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="gauss1m", choices=sorted(WORKLOADS) + ["batch256", "serve", "copy256m"])
+    ap.add_argument("--workload", default="gauss1m",
+                    choices=sorted(WORKLOADS) + ["batch256", "serve", "copy256m", "pingpong"])
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
+    ap.add_argument("--extras", default="on", choices=["on", "off"],
+                    help="gauss1m: also run the serve and batch256 legs into the same line")
     ap.add_argument("--autotune", default="off", choices=["on", "off"],
-                    help="time every kernel shape on this cloud first (off: the planner's "
-                         "per-pass time model, which picks the same shape for gauss1m)")
+                    help="time every plan on this cloud first (off: the planner's choice)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -398,6 +600,9 @@ def main() -> int:
         return 0
     if args.workload == "copy256m":
         print(json.dumps(run_copy(args)), flush=True)
+        return 0
+    if args.workload == "pingpong":
+        print(json.dumps(run_pingpong(args)), flush=True)
         return 0
 
     from fpsample_amd import device as D
@@ -420,7 +625,7 @@ def main() -> int:
         soa = np.ascontiguousarray(pc.T)  # (3, N): coordinate j of point i at j*N + i
         d_pts = D.DeviceArray.from_numpy(soa)
         if args.autotune == "on":
-            D.autotune(d_pts, 1, n, n, 3)  # untimed: picks the kernel shape for this cloud
+            D.autotune(d_pts, 1, n, n, 3, passes=m)  # untimed: picks the plan for this cloud
         plan = D.plan_info(n, 3, n_samples=m)
 
         def step():
@@ -449,16 +654,10 @@ def main() -> int:
     ms_per_step = wall / args.steps * 1e3
     value = m * args.steps / wall
     kernel_s = kernel_ms / max(launches, 1) / 1e3
-    if npdu:
-        # per pass: the window's w+1 points updated (20 B each) and the argmax
-        # over every min-distance (4 B per point), src/lib.cpp:302-315
-        alg_bytes = (20 * (min(w, n - 1) + 1) + 4 * n) * (m - 1)
-    else:
-        alg_bytes = BYTES_PER_POINT_PASS * n * (m - 1)
-    achieved = alg_bytes / kernel_s / 1e9
+    # right after the timed launches: the last launch's exchange count, then the probe
+    roof = roofline(args.workload, plan, n, m, kernel_s, npdu, w)
     result = {
-        "metric": "samples/s at N=1M,M=16K (+% roofline)" if args.workload == "gauss1m"
-        else f"samples/s ({args.workload})",
+        "metric": METRIC if args.workload == "gauss1m" else f"samples/s ({args.workload})",
         "value": round(value, 3),
         "unit": "samples/s",
         "n_gpus": 1,
@@ -472,18 +671,19 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wl["desc"], "n_points": n, "n_samples": m, "dim": 3,
                    "plan": plan},
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _read_traffic(args.workload, plan),
-            "kernel_ms": round(kernel_s * 1e3, 4),
-            "algorithmic_bytes_per_launch": alg_bytes,
-        },
+        "roofline": roof,
         "cpu_baseline": None,
     }
+    if not npdu and args.workload in ("gauss1m", "lattice16m"):
+        ref = None
+        try:
+            from tests.golden import full_length as FL
+
+            if FL.available(args.workload):
+                ref = bool(np.array_equal(out, FL.load(args.workload)["out"]))
+        except ImportError:
+            pass
+        result["identical_to_reference"] = ref
     if args.cpu_baseline == "on":
         result["cpu_baseline"] = cpu_baseline(pc, m, args.cpu_seconds, w)
         if args.workload == "gauss1m":
@@ -492,6 +692,9 @@ def main() -> int:
             from oracle import oracle as O
 
             result["accuracy_vs_fp64"] = O.accuracy_vs_f64(pc, out, 0)
+    if args.workload == "gauss1m" and args.extras == "on":
+        result["serve"] = run_serve(args)
+        result["batch256"] = run_batch(args)
     print(json.dumps(result), flush=True)
     return 0
 

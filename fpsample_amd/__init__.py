@@ -18,6 +18,14 @@ from typing import List, Optional, Union
 
 import numpy as np
 
+from .device import hip as _load_hip_runtime
+
+# The HIP runtime is loaded under its unversioned name before the extension
+# (whose libfpsx.so needs it): a torch imported later then binds to this same
+# runtime instead of mapping its bundled copy beside it, and a torch imported
+# earlier has its copy reused here.  One runtime per process either way.
+_load_hip_runtime()
+
 try:
     from ._fpsample import (  # noqa: F401
         __doc__ as _ext_doc,

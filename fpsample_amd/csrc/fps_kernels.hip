@@ -311,6 +311,11 @@ __global__ void fill_f32(float *p, size_t n, float v) {
     p[i] = v;
 }
 
+// acc |= src: folds one launch's status word into a word that outlives it.
+__global__ void or_u32(const uint32_t *src, uint32_t *acc) {
+  if (threadIdx.x == 0) acc[0] |= src[0];
+}
+
 // Strided (point stride sp, coordinate stride sd) -> SoA (stride ld) copy.
 __global__ void strided_to_soa(const float *__restrict__ src, uint64_t sp, uint64_t sd,
                                float *__restrict__ soa, uint32_t n, uint32_t dim, uint64_t ld) {
@@ -372,6 +377,10 @@ hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t st) {
   if (n == 0) return hipSuccess;
   unsigned blocks = (unsigned)min((size_t)2048, (n + 255) / 256);
   hipLaunchKernelGGL(fill_f32, dim3(blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_or_u32(const uint32_t *src, uint32_t *acc, hipStream_t st) {
+  hipLaunchKernelGGL(or_u32, dim3(1), dim3(64), 0, st, src, acc);
   return hipGetLastError();
 }
 hipError_t launch_strided_to_soa(const float *src, uint64_t sp, uint64_t sd, float *soa,

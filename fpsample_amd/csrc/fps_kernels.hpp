@@ -50,6 +50,8 @@ const KernelEntry *stream_table(size_t *count);
 // Enqueue helpers; each returns the launch status.
 hipError_t launch_fill_u64(uint64_t *p, size_t n, uint64_t v, hipStream_t st);
 hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t st);
+// acc[0] |= src[0] on the stream (one status word folded into another).
+hipError_t launch_or_u32(const uint32_t *src, uint32_t *acc, hipStream_t st);
 // Copy a strided point array (coordinate j of point i at src[i*sp + j*sd])
 // into SoA layout (soa[j*ld + i]).
 hipError_t launch_strided_to_soa(const float *src, uint64_t sp, uint64_t sd, float *soa,

@@ -673,6 +673,9 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
     }
     s += have2 ? 2u : 1u;
   }
+  // The launch's exchange count (the box exchange included) in the second
+  // status word: how many rounds the pairing saved is data-dependent.
+  if (recorder && lane == 0) f.status[1] = e;
 }
 
 // ---------------------------------------------------------------------------

@@ -75,6 +75,14 @@ DevBuf::~DevBuf() {
   // Device memory is released with the process; contexts live until exit.
 }
 
+// Realtime-clock ticks an exchange may wait: FPSX_EXCHANGE_TIMEOUT_S, else 2 s.
+static uint64_t default_spin_ticks(int khz) {
+  const char *sec = std::getenv("FPSX_EXCHANGE_TIMEOUT_S");
+  double secs = sec ? std::atof(sec) : 2.0;
+  if (!(secs > 0)) secs = 2.0;
+  return (uint64_t)((double)khz * 1000.0 * secs);
+}
+
 static std::mutex g_ctx_mu;
 static std::map<int, std::unique_ptr<Ctx>> g_ctx;
 
@@ -95,10 +103,8 @@ int get_ctx(Ctx **out) {
   int khz = 0;
   HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
   if (khz <= 0) khz = 100000;
-  const char *sec = std::getenv("FPSX_EXCHANGE_TIMEOUT_S");
-  double secs = sec ? std::atof(sec) : 2.0;
-  if (!(secs > 0)) secs = 2.0;
-  c->spin_ticks = (uint64_t)((double)khz * 1000.0 * secs);
+  c->clock_khz = khz;
+  c->spin_ticks = default_spin_ticks(khz);
 
   HIP_TRY(hipHostMalloc((void **)&c->host_status, sizeof(uint32_t), hipHostMallocDefault));
   *out = c.get();
@@ -241,9 +247,8 @@ static double model_us(size_t n, size_t g) {
 static int plan_from_tuned(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
   auto it = c.tuned.find(std::make_tuple(n, dim, tie));
   if (it == c.tuned.end()) return -1;
-  const KernelEntry *k = it->second.first;
-  if (k->ppt > 0) return plan_resident_shape(c, n, k, p) == FPSX_OK ? FPSX_OK : -1;
-  return plan_stream(c, n, dim, tie, k->block, it->second.second, k->dim == 0, p);
+  p = it->second;
+  return FPSX_OK;
 }
 
 int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
@@ -522,21 +527,38 @@ int plan_npdu(Ctx &c, size_t n, size_t dim, Plan &p) {
   return plan_npdu_stream(c, n, dim, 0, p);
 }
 
-// Candidate plans the tuner times for a cloud: every feasible resident shape
-// of this dimension plus the streaming kernel at two block sizes.
-static std::vector<Plan> candidates(Ctx &c, size_t n, size_t dim, int tie) {
+// Candidate plans the tuner times for a cloud of n points and m samples:
+// the planner's own choice, every feasible resident shape of this dimension,
+// for 3-D clouds every feasible pruned and paired shape and the streaming
+// pruned plan, and the streaming kernel at two block sizes.
+static std::vector<Plan> candidates(Ctx &c, size_t n, size_t dim, int tie, size_t m) {
   std::vector<Plan> out;
+  Plan q;
+  if (plan(c, n, dim, tie, m, q) == FPSX_OK) out.push_back(q);
   size_t cnt;
   const KernelEntry *t = resident_table(&cnt);
   for (size_t i = 0; i < cnt; ++i) {
     if (t[i].dim != (int)dim || t[i].tie != tie) continue;
-    Plan q;
     if (plan_resident_shape(c, n, &t[i], q) == FPSX_OK) out.push_back(q);
   }
+  if (dim == 3) {
+    t = pruned_table(&cnt);
+    for (size_t i = 0; i < cnt; ++i)
+      if (t[i].dim == 3 && t[i].tie == tie &&
+          plan_resident_shape(c, n, &t[i], q, PlanKind::Pruned) == FPSX_OK)
+        out.push_back(q);
+    t = pair_table(&cnt);
+    for (size_t i = 0; i < cnt; ++i)
+      if (t[i].dim == 3 && t[i].tie == tie &&
+          plan_resident_shape(c, n, &t[i], q, PlanKind::Pair) == FPSX_OK && q.grid >= 2 &&
+          q.grid <= kPairMaxGrid)
+        out.push_back(q);
+    if (plan_stream_pruned(c, n, tie, 0, q) == FPSX_OK) out.push_back(q);
+  }
   for (int B : {256, 1024}) {
-    Plan q;
     if (plan_stream(c, n, dim, tie, B, 0, false, q) == FPSX_OK) out.push_back(q);
   }
+  t_err.clear();  // infeasible candidates are skipped, not reported
   return out;
 }
 
@@ -619,6 +641,8 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
                size_t dim, size_t m, const uint64_t *d_starts, size_t n_starts, uint64_t *d_out,
                float *d_out_pts, float init, hipStream_t st) {
   if (p.kind == PlanKind::Npdu) return fail(FPSX_EINVAL, "launch_fps: windowed-update plan");
+  int rc = scratch_acquire(c, st);
+  if (rc) return rc;
   if (p.kind == PlanKind::Stream && sp != 1) {
     // The streaming kernel re-reads every point on every pass: give it SoA.
     HIP_TRY(c.pts.reserve(n * dim * sizeof(float)));
@@ -628,8 +652,8 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
     ld = n;
   }
   FpsParams a;
-  int rc = common_params(c, d_pts, sp, ld, n, dim, m, d_starts, n_starts, d_out, d_out_pts, init,
-                         p.chunk, st, a);
+  rc = common_params(c, d_pts, sp, ld, n, dim, m, d_starts, n_starts, d_out, d_out_pts, init,
+                     p.chunk, st, a);
   if (rc) return rc;
   if (p.kind == PlanKind::Stream) {
     HIP_TRY(c.dmin.reserve(n * sizeof(float)));
@@ -644,8 +668,8 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
     if (!c.spatial) c.spatial = new_spatial_scratch();
     const uint32_t block =
         (p.kind == PlanKind::StreamPruned) ? (uint32_t)kSBlock : (uint32_t)(64 * p.k->ppt);
-    int rc = spatial_order(d_pts, sp, ld, (uint32_t)n, block, (float *)c.sorted.p,
-                           (uint32_t *)c.perm.p, *c.spatial, st);
+    rc = spatial_order(d_pts, sp, ld, (uint32_t)n, block, (float *)c.sorted.p,
+                       (uint32_t *)c.perm.p, *c.spatial, st);
     if (rc) return rc;
     if (p.kind == PlanKind::StreamPruned) {
       // Min-distances by position; the first pass writes every one.
@@ -658,7 +682,12 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
     pa.perm = (const uint32_t *)c.perm.p;
     arg = &pa;
   }
-  return dispatch(c, p, arg, st);
+  rc = dispatch(c, p, arg, st);
+  if (rc) return rc;
+  c.last_plan = p;
+  c.last_m = m;
+  c.last_starts = n_starts;
+  return scratch_release(c, st);
 }
 
 // Enqueue one windowed-update run (single start d_start[0]).  Caller holds c.mu.
@@ -666,8 +695,10 @@ int launch_npdu(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld,
                 size_t dim, size_t m, size_t window, const uint64_t *d_start, uint64_t *d_out,
                 float *d_out_pts, hipStream_t st) {
   if (p.kind != PlanKind::Npdu) return fail(FPSX_EINVAL, "launch_npdu: not a windowed-update plan");
+  int rc = scratch_acquire(c, st);
+  if (rc) return rc;
   NpduParams a;
-  int rc = common_params(c, d_pts, sp, ld, n, dim, m, d_start, 1, d_out, d_out_pts, INFINITY,
+  rc = common_params(c, d_pts, sp, ld, n, dim, m, d_start, 1, d_out, d_out_pts, INFINITY,
                          p.chunk, st, a.fps);
   if (rc) return rc;
   // Any window of 2n or more covers the whole cloud; the clamp keeps the
@@ -678,7 +709,29 @@ int launch_npdu(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld,
     HIP_TRY(c.dmin.reserve(n * sizeof(float)));
     a.fps.dmin = (float *)c.dmin.p;
   }
-  return dispatch(c, p, &a, st);
+  rc = dispatch(c, p, &a, st);
+  if (rc) return rc;
+  return scratch_release(c, st);
+}
+
+static bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+int scratch_acquire(Ctx &c, hipStream_t st) {
+  if (!c.scratch_recorded || c.scratch_owner == st || capturing(st)) return FPSX_OK;
+  HIP_TRY(hipStreamWaitEvent(st, c.scratch_ev, 0));
+  return FPSX_OK;
+}
+
+int scratch_release(Ctx &c, hipStream_t st) {
+  if (capturing(st)) return FPSX_OK;
+  if (!c.scratch_ev) HIP_TRY(hipEventCreateWithFlags(&c.scratch_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(c.scratch_ev, st));
+  c.scratch_owner = st;
+  c.scratch_recorded = true;
+  return FPSX_OK;
 }
 
 int status_code(uint32_t st) {
@@ -700,6 +753,8 @@ int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
   rc = plan(c, n, dim, tie, m, p);
   if (rc) return rc;
   hipStream_t st = c.stream;
+  rc = scratch_acquire(c, st);
+  if (rc) return rc;
   const size_t bytes = n * dim * sizeof(float);
   HIP_TRY(c.pts.reserve(bytes));
   HIP_TRY(c.starts.reserve(n_starts * sizeof(uint64_t)));
@@ -846,6 +901,8 @@ FPSX_EXPORT int fpsx_fps_npdu_sampling(const float *points, size_t n_points, siz
   rc = plan_npdu(c, n_points, dim, p);
   if (rc) return rc;
   hipStream_t st = c.stream;
+  rc = scratch_acquire(c, st);
+  if (rc) return rc;
   const size_t bytes = n_points * dim * sizeof(float);
   HIP_TRY(c.pts.reserve(bytes));
   HIP_TRY(c.starts.reserve(sizeof(uint64_t)));
@@ -882,6 +939,38 @@ FPSX_EXPORT int fpsx_npdu_plan_info(size_t n_points, size_t dim, int *kernel_dim
   return FPSX_OK;
 }
 
+FPSX_EXPORT int fpsx_launch_exchanges(void *stream, uint64_t *exchanges) {
+  t_err.clear();
+  if (!exchanges) return fail(FPSX_EINVAL, "null pointer argument");
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  Ctx &c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  *exchanges = 0;
+  if (!c.last_plan.k || c.last_plan.grid < 2) return FPSX_OK;
+  if (c.last_plan.kind != PlanKind::Pair) {
+    // One exchange per pick after the forced prefix.
+    *exchanges = c.last_m > c.last_starts ? c.last_m - c.last_starts : 0;
+    return FPSX_OK;
+  }
+  hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+  rc = scratch_acquire(c, st);
+  if (rc) return rc;
+  uint32_t words[2] = {0, 0};
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipMemcpy(words, c.status.p, sizeof(words), hipMemcpyDeviceToHost));
+  *exchanges = words[1];
+  return FPSX_OK;
+}
+
+FPSX_EXPORT void *fpsx_stream(void) {
+  t_err.clear();
+  Ctx *cp = nullptr;
+  if (get_ctx(&cp) != FPSX_OK) return nullptr;
+  return (void *)cp->stream;
+}
+
 FPSX_EXPORT int fpsx_device_status(void *stream) {
   t_err.clear();
   Ctx *cp = nullptr;
@@ -891,6 +980,8 @@ FPSX_EXPORT int fpsx_device_status(void *stream) {
   std::lock_guard<std::mutex> lk(c.mu);
   hipStream_t st = stream ? (hipStream_t)stream : c.stream;
   if (!c.status.p) return FPSX_OK;
+  rc = scratch_acquire(c, st);
+  if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(c.host_status, c.status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return status_code(*c.host_status);
@@ -928,10 +1019,13 @@ FPSX_EXPORT int fpsx_autotune(const float *d_points, size_t stride_point, size_t
   std::lock_guard<std::mutex> lk(c.mu);
   hipStream_t st = stream ? (hipStream_t)stream : c.stream;
   const size_t m = std::max<size_t>(2, std::min(passes ? passes : 256, n_points));
+  rc = scratch_acquire(c, st);
+  if (rc) return rc;
   HIP_TRY(c.starts.reserve(sizeof(uint64_t)));
   HIP_TRY(c.out.reserve(m * sizeof(uint64_t)));
   HIP_TRY(launch_fill_u64((uint64_t *)c.starts.p, 1, 0, st));
-  std::vector<Plan> cands = candidates(c, n_points, dim, tie);
+  c.tuned.erase(std::make_tuple(n_points, dim, tie));  // the planner's own choice is a candidate
+  std::vector<Plan> cands = candidates(c, n_points, dim, tie, m);
   if (cands.empty()) return fail(FPSX_EINVAL, "no feasible plan");
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
@@ -967,10 +1061,21 @@ FPSX_EXPORT int fpsx_autotune(const float *d_points, size_t stride_point, size_t
     if (!bp || cands[ci].grid < bp->grid) bp = &cands[ci];
   }
   if (!bp) return fail(FPSX_EHIP, "every candidate plan failed");
-  c.tuned[std::make_tuple(n_points, dim, tie)] = std::make_pair(bp->k, bp->grid);
+  c.tuned[std::make_tuple(n_points, dim, tie)] = *bp;
   if (chosen_block) *chosen_block = (int)bp->block;
   if (chosen_ppt) *chosen_ppt = bp->k->ppt;
   if (chosen_grid) *chosen_grid = (int)bp->grid;
+  return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_exchange_timeout(double seconds) {
+  t_err.clear();
+  Ctx *cp = nullptr;
+  int rc = get_ctx(&cp);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(cp->mu);
+  cp->spin_ticks = seconds > 0 ? std::max<uint64_t>(1, (uint64_t)((double)cp->clock_khz * 1000.0 * seconds))
+                               : default_spin_ticks(cp->clock_khz);
   return FPSX_OK;
 }
 

@@ -313,7 +313,14 @@ FPSX_EXPORT int fpsx_batch_run(fpsx_batch *b, void *stream) {
   std::lock_guard<std::mutex> lk(c.mu);
   hipStream_t st = stream ? (hipStream_t)stream : c.stream;
   if (b->fallback) {
-    // No batch kernel for this dimension: one planned launch per cloud.
+    // No batch kernel for this dimension or cloud size: one planned launch
+    // per cloud.  Every launch clears the context's status word, so each
+    // one's status is folded into the batch's own word, which the batch
+    // status then reports (a failure in any cloud, not only the last).
+    HIP_TRY(b->run.d_status.reserve(sizeof(uint64_t)));
+    int rc = scratch_acquire(c, st);
+    if (rc) return rc;
+    HIP_TRY(launch_fill_u64((uint64_t *)b->run.d_status.p, 1, 0, st));
     HIP_TRY(c.starts.reserve(b->specs.size() * sizeof(uint64_t)));
     std::vector<uint64_t> starts(b->specs.size());
     for (size_t i = 0; i < starts.size(); ++i) starts[i] = b->specs[i].start;
@@ -322,14 +329,15 @@ FPSX_EXPORT int fpsx_batch_run(fpsx_batch *b, void *stream) {
     for (size_t i = 0; i < b->specs.size(); ++i) {
       const JobSpec &s = b->specs[i];
       Plan p;
-      int rc = plan(c, s.n, b->dim, kTieHighest, s.m, p);
+      rc = plan(c, s.n, b->dim, kTieHighest, s.m, p);
       if (rc) return rc;
       rc = launch_fps(c, p, b->d_pts + s.pt_off, 1, b->ld, s.n, b->dim, s.m,
                       (const uint64_t *)c.starts.p + i, 1, b->d_out + s.out_off, nullptr,
                       INFINITY, st);
       if (rc) return rc;
+      HIP_TRY(launch_or_u32((const uint32_t *)c.status.p, (uint32_t *)b->run.d_status.p, st));
     }
-    return FPSX_OK;
+    return scratch_release(c, st);
   }
   if (b->exec) {
     HIP_TRY(hipGraphLaunch(b->exec, st));
@@ -344,11 +352,8 @@ FPSX_EXPORT int fpsx_batch_status(fpsx_batch *b, void *stream) {
   std::lock_guard<std::mutex> lk(c.mu);
   hipStream_t st = stream ? (hipStream_t)stream : c.stream;
   HIP_TRY(hipStreamSynchronize(st));
-  if (b->fallback) {
-    HIP_TRY(hipMemcpy(c.host_status, c.status.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
-  } else {
-    HIP_TRY(hipMemcpy(c.host_status, b->run.d_status.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
-  }
+  if (!b->run.d_status.p) return FPSX_OK;  // never run
+  HIP_TRY(hipMemcpy(c.host_status, b->run.d_status.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
   return status_code(*c.host_status);
 }
 

@@ -236,7 +236,11 @@ py::list fps_sampling_batch_py(py::list clouds, std::vector<size_t> n_samples,
 
 }  // namespace
 
-PYBIND11_MODULE(_fpsample, m) {
+// Declared as the reference declares its module (src/lib.cpp:581): safe
+// without the GIL (the device calls release it anyway and the C ABI is
+// thread-safe) and under per-interpreter GILs (no static Python state).
+PYBIND11_MODULE(_fpsample, m, py::mod_gil_not_used(),
+                py::multiple_interpreters::per_interpreter_gil()) {
   m.doc() = R"pbdoc(
         MI355X farthest point sampling (FPS)
         -----------------------------------

@@ -27,6 +27,15 @@ _fpsx = None
 
 
 def hip() -> ctypes.CDLL:
+    """The HIP runtime libfpsx.so runs on, loaded under its unversioned name.
+
+    Loading it by that name before libfpsx.so matters when torch shares the
+    process: torch's libraries ask for "libamdhip64.so" and reuse an object
+    already loaded under that name, while libfpsx.so's "libamdhip64.so.7"
+    matches the soname of whichever copy is loaded first (the system's, or
+    torch's bundled one when torch came first).  Without it a torch imported
+    after this library maps a second HIP runtime, which then sees no device.
+    """
     global _hip
     if _hip is None:
         for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
@@ -53,6 +62,7 @@ def _check(rc: int, what: str) -> None:
 def lib() -> ctypes.CDLL:
     global _fpsx
     if _fpsx is None:
+        hip()  # the runtime first, under the name torch asks for
         L = ctypes.CDLL(os.path.join(HERE, "libfpsx.so"))
         sz, vp = ctypes.c_size_t, ctypes.c_void_p
         L.fpsx_last_error.restype = ctypes.c_char_p
@@ -63,6 +73,8 @@ def lib() -> ctypes.CDLL:
                                                        ctypes.c_int, ctypes.c_float, vp]
         L.fpsx_fps_sampling_device_strided.restype = ctypes.c_int
         L.fpsx_device_status.argtypes = [vp]
+        L.fpsx_stream.restype = vp
+        L.fpsx_stream.argtypes = []
         ip_ = ctypes.POINTER(ctypes.c_int)
         L.fpsx_autotune.argtypes = [vp, sz, sz, sz, sz, ctypes.c_int, sz, vp, ip_, ip_, ip_]
         L.fpsx_autotune.restype = ctypes.c_int
@@ -174,51 +186,189 @@ def npdu_plan_info(n_points: int, dim: int) -> dict:
     return dict(kernel_dim=kdim, block=block, ppt=ppt, grid=grid)
 
 
-def _tensor_view(t):
-    """(address, shape, strides in elements) of a 2-D float32 device array:
-    a torch tensor on the GPU, or anything exposing __cuda_array_interface__."""
-    if hasattr(t, "data_ptr") and hasattr(t, "stride"):
-        if str(getattr(t, "dtype", "")) not in ("torch.float32", "float32"):
+class _DLDevice(ctypes.Structure):
+    _fields_ = [("device_type", ctypes.c_int32), ("device_id", ctypes.c_int32)]
+
+
+class _DLDataType(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_uint8), ("bits", ctypes.c_uint8), ("lanes", ctypes.c_uint16)]
+
+
+class _DLTensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("device", _DLDevice), ("ndim", ctypes.c_int32),
+                ("dtype", _DLDataType), ("shape", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
+
+
+class _DLManagedTensor(ctypes.Structure):
+    pass
+
+
+_DLManagedTensor._fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", ctypes.c_void_p),
+                             ("deleter", ctypes.CFUNCTYPE(None, ctypes.POINTER(_DLManagedTensor)))]
+
+_KDL_ROCM = 10  # DLPack device type of ROCm device memory
+_KDL_FLOAT = 2
+
+
+class _DLPackHold:
+    """Owns a consumed DLPack capsule; releases the producer's tensor on close()."""
+
+    def __init__(self, capsule):
+        api = ctypes.pythonapi
+        api.PyCapsule_GetPointer.restype = ctypes.c_void_p
+        api.PyCapsule_GetPointer.argtypes = [ctypes.py_object, ctypes.c_char_p]
+        api.PyCapsule_SetName.argtypes = [ctypes.py_object, ctypes.c_char_p]
+        ptr = api.PyCapsule_GetPointer(capsule, b"dltensor")
+        if not ptr:
+            raise TypeError("not a DLPack tensor capsule")
+        # Renamed so the capsule's own destructor leaves the tensor to us.
+        api.PyCapsule_SetName(capsule, b"used_dltensor")
+        self.mt = ctypes.cast(ptr, ctypes.POINTER(_DLManagedTensor))
+
+    def close(self) -> None:
+        if self.mt is not None:
+            if self.mt.contents.deleter:
+                self.mt.contents.deleter(self.mt)
+            self.mt = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _is_torch(t) -> bool:
+    return type(t).__module__.split(".")[0] == "torch" and hasattr(t, "is_cuda")
+
+
+def _tensor_view(t, stream: Optional[int] = None):
+    """(address, shape, strides in elements, keep-alive) of a 2-D float32
+    device array: a torch tensor on the GPU, an object exporting DLPack
+    (__dlpack__ / __dlpack_device__) on a ROCm device, or anything exposing
+    __cuda_array_interface__.  Host tensors are refused here, before any
+    launch.  For DLPack the producer is asked to order its pending writes
+    before `stream` (the stream the caller's FPS will run on)."""
+    if _is_torch(t):
+        if not t.is_cuda:
+            raise TypeError(f"points must be a GPU tensor (got device {t.device})")
+        if str(t.dtype) != "torch.float32":
             raise TypeError("points must be float32")
-        return int(t.data_ptr()), tuple(t.shape), tuple(t.stride())
+        if t.dim() != 2:
+            raise ValueError("points must be a 2-D (n_points, dim) tensor")
+        return int(t.data_ptr()), tuple(t.shape), tuple(t.stride()), None
+    if hasattr(t, "__dlpack__") and hasattr(t, "__dlpack_device__"):
+        dev_type, _ = t.__dlpack_device__()
+        if int(dev_type) != _KDL_ROCM:
+            raise TypeError(f"points must be in ROCm device memory (DLPack device type {dev_type})")
+        hold = _DLPackHold(t.__dlpack__(stream=stream if stream is not None else lib_stream()))
+        dl = hold.mt.contents.dl_tensor
+        if dl.dtype.code != _KDL_FLOAT or dl.dtype.bits != 32 or dl.dtype.lanes != 1:
+            hold.close()
+            raise TypeError("points must be float32")
+        if dl.ndim != 2:
+            hold.close()
+            raise ValueError("points must be a 2-D (n_points, dim) tensor")
+        shape = (int(dl.shape[0]), int(dl.shape[1]))
+        strides = ((int(dl.strides[0]), int(dl.strides[1])) if dl.strides
+                   else (shape[1], 1))
+        return int(dl.data or 0) + int(dl.byte_offset), shape, strides, hold
     cai = getattr(t, "__cuda_array_interface__", None)
     if cai is None:
-        raise TypeError("expected a device tensor (torch) or an object with __cuda_array_interface__")
+        raise TypeError("expected a GPU tensor (torch), a DLPack-exporting device array or an "
+                        "object with __cuda_array_interface__")
     if np.dtype(cai["typestr"]) != np.float32:
         raise TypeError("points must be float32")
     shape = tuple(cai["shape"])
+    if len(shape) != 2:
+        raise ValueError("points must be a 2-D (n_points, dim) tensor")
     strides = cai.get("strides")
     if strides is None:
         strides = (shape[1] * 4, 4)
-    return int(cai["data"][0]), shape, tuple(s // 4 for s in strides)
+    return int(cai["data"][0]), shape, tuple(s // 4 for s in strides), None
+
+
+def lib_stream() -> int:
+    """The library's own stream for the current device (a hipStream_t as int)."""
+    return int(lib().fpsx_stream() or 0)
 
 
 def fps_sampling_tensor(points, n_samples: int, start_idx: Union[int, Sequence[int]] = 0,
-                        return_points: bool = False, stream: Optional[int] = None):
-    """FPS on a device-resident (n_points, dim) float32 tensor, used in place.
+                        return_points: bool = False, stream: Optional[int] = None,
+                        sync: bool = True):
+    """FPS on a device-resident (n_points, dim) float32 tensor, used in place
+    (the zero-copy form of fps_sampling; src/fpsample/__init__.py:62 hands the
+    reference an F-order float32 copy instead).
 
-    Returns (indices, coordinates-or-None) as DeviceArrays: indices uint64
-    (n_samples,), coordinates float32 (n_samples, dim) when return_points.
-    The work is enqueued on `stream`; call synchronize() (or read the arrays,
-    which synchronises) before use.
+    * torch tensor on the GPU: the work runs on the caller's stream
+      (torch.cuda.current_stream(), or `stream`), after whatever torch work
+      produced the tensor; returns torch tensors: indices int64 (n_samples,),
+      coordinates float32 (n_samples, dim) when return_points, else None.
+    * an object exporting DLPack on a ROCm device: its producer orders its
+      writes before the library's stream (or `stream`); returns DeviceArrays
+      (indices uint64).
+    * an object with __cuda_array_interface__: returns DeviceArrays.
+
+    sync=True (default) waits for the run and raises if the kernel reported a
+    failure (an exchange time-out); with sync=False the call returns once the
+    work is enqueued, and device_status(stream) reports the status later.
     """
-    addr, shape, strides = _tensor_view(points)
-    if len(shape) != 2:
-        raise ValueError("points must be a 2-D (n_points, dim) tensor")
-    n, d = shape
-    starts = np.atleast_1d(np.asarray(start_idx, dtype=np.uint64))
-    if n_samples < 1 or n_samples > n:
-        raise ValueError("n_samples must be in [1, n_points]")
-    if starts.size < 1 or starts.size > n_samples or np.any(starts >= n):
-        raise ValueError("start indices must be < n_points, 1 <= count <= n_samples")
-    d_start = DeviceArray.from_numpy(starts)
-    out = DeviceArray((n_samples,), np.uint64)
-    out_pts = DeviceArray((n_samples, d), np.float32) if return_points else None
-    fps_sampling_device_strided(addr, strides[0], strides[1], n, d, n_samples, d_start,
-                                starts.size, out, out_pts, stream=stream)
-    synchronize()
-    d_start.free()
+    torch_in = _is_torch(points)
+    if torch_in:
+        import torch
+
+        if points.is_cuda:
+            dev = points.device
+            producer = torch.cuda.current_stream(dev)
+            if stream is None:
+                ts = producer
+            else:
+                # A stream of the caller's: order it after the producer's work,
+                # and allocate the outputs for it.
+                ts = torch.cuda.ExternalStream(int(stream), device=dev)
+                ts.wait_stream(producer)
+            stream = int(ts.cuda_stream)
+    addr, shape, strides, hold = _tensor_view(points, stream)
+    try:
+        n, d = shape
+        starts = np.atleast_1d(np.asarray(start_idx, dtype=np.uint64))
+        if n_samples < 1 or n_samples > n:
+            raise ValueError("n_samples must be in [1, n_points]")
+        if starts.size < 1 or starts.size > n_samples or np.any(starts >= n):
+            raise ValueError("start indices must be < n_points, 1 <= count <= n_samples")
+        if torch_in:
+            with torch.cuda.device(dev), torch.cuda.stream(ts):
+                d_start = torch.as_tensor(starts.astype(np.int64)).to(dev, non_blocking=False)
+                out = torch.empty(n_samples, dtype=torch.int64, device=dev)
+                out_pts = (torch.empty((n_samples, d), dtype=torch.float32, device=dev)
+                           if return_points else None)
+                fps_sampling_device_strided(addr, strides[0], strides[1], n, d, n_samples,
+                                            d_start, starts.size, out, out_pts, stream=stream)
+        else:
+            d_start = DeviceArray.from_numpy(starts)
+            out = DeviceArray((n_samples,), np.uint64)
+            out_pts = DeviceArray((n_samples, d), np.float32) if return_points else None
+            fps_sampling_device_strided(addr, strides[0], strides[1], n, d, n_samples, d_start,
+                                        starts.size, out, out_pts, stream=stream)
+            # The start array and the producer's tensor must outlive the run:
+            # freeing `out` (hipFree waits for the device) releases them after it.
+            out._keep = (d_start, hold)
+        if sync:
+            device_status(stream)
+    except BaseException:
+        if hold is not None:
+            _synchronize_quietly()
+            hold.close()
+        raise
     return out, out_pts
+
+
+def _synchronize_quietly() -> None:
+    try:
+        synchronize()
+    except Exception:
+        pass
 
 
 def autotune(points, stride_point: int, stride_dim: int, n_points: int, dim: int,
@@ -235,6 +385,14 @@ def autotune(points, stride_point: int, stride_dim: int, n_points: int, dim: int
 def device_status(stream: Optional[int] = None) -> None:
     """Wait for the stream and raise if the last FPS kernel reported a failure."""
     _raise(lib().fpsx_device_status(stream), "fpsx_device_status")
+
+
+def exchange_timeout(seconds: float) -> None:
+    """Bound on one cross-workgroup exchange wait for later launches on the
+    current device (seconds <= 0: the default, 2 s or FPSX_EXCHANGE_TIMEOUT_S)."""
+    L = lib()
+    L.fpsx_exchange_timeout.argtypes = [ctypes.c_double]
+    _raise(L.fpsx_exchange_timeout(float(seconds)), "fpsx_exchange_timeout")
 
 
 def synchronize() -> None:

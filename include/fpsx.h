@@ -58,6 +58,16 @@ int fpsx_device_count(void);
 int fpsx_device_status(void *stream);
 
 /*
+ * The library's own stream for the current device (a hipStream_t; NULL when
+ * no device is usable): what a NULL `stream` argument means.  A caller
+ * handing the library a tensor still being written on another stream orders
+ * that work before this one (e.g. the `stream` argument of DLPack's
+ * __dlpack__).  Calls on different streams are ordered by the library
+ * wherever they share its scratch buffers.
+ */
+void *fpsx_stream(void);
+
+/*
  * The launch the planner picks for a cloud of n_points x dim and n_samples
  * picks under `tie`: kind 0 = register-resident, 1 = HBM-streaming,
  * 2 = register-resident with spatial order and per-wave pruning,
@@ -74,17 +84,51 @@ int fpsx_plan_info(size_t n_points, size_t dim, int tie, size_t n_samples, int *
                    int *ppt, int *grid, int *chunk);
 
 /*
- * Autotuner: times every feasible kernel shape (threads per workgroup, points
- * per thread, workgroups) for `passes` passes (0: 256) on the given device
- * cloud and keeps, among the shapes within 3% of the fastest, the one with
- * the fewest workgroups, for clouds of exactly this (n_points, dim, tie) on
- * the current device; later calls plan with it.  The chosen shape is
- * returned through the optional pointers.  Without tuning the planner uses a
- * per-pass time model.
+ * Autotuner: times every feasible plan for `passes` passes (0: 256; pass the
+ * number of samples later calls will draw) on the given device cloud -- the
+ * planner's own choice, every resident shape (threads per workgroup, points
+ * per thread, workgroups), for 3-D clouds every pruned and paired shape and
+ * the streaming pruned plan, and the streaming kernel -- and keeps, among the
+ * plans within 3% of the fastest, the one with the fewest workgroups, for
+ * clouds of exactly this (n_points, dim, tie) on the current device; later
+ * calls use it whatever their sample count.  The chosen shape is returned
+ * through the optional pointers (fpsx_plan_info reports its kind).  Without
+ * tuning the planner uses its per-pass time model.
  */
 int fpsx_autotune(const float *d_points, size_t stride_point, size_t stride_dim, size_t n_points,
                   size_t dim, int tie, size_t passes, void *stream, int *chosen_block,
                   int *chosen_ppt, int *chosen_grid);
+
+/*
+ * Bound on one cross-workgroup exchange wait for later launches on the
+ * current device (FPS, windowed-update and batch objects created afterwards):
+ * a kernel whose exchange waits longer gives up and reports FPSX_ETIMEOUT.
+ * seconds <= 0 restores the default (FPSX_EXCHANGE_TIMEOUT_S, else 2 s).
+ * No counterpart in the reference (its loop is single-threaded).
+ */
+int fpsx_exchange_timeout(double seconds);
+
+/*
+ * Number of cross-workgroup exchanges the last FPS launch on the current
+ * device made (after `stream`, NULL: the library's, reaches it): 0 for a
+ * one-workgroup plan, one per pick after the forced prefix for the one-pick
+ * kernels, counted by the kernel for the paired kind (up to two picks per
+ * exchange, plus one exchange of workgroup boxes).  Measurement aid; no
+ * counterpart in the reference.
+ */
+int fpsx_launch_exchanges(void *stream, uint64_t *exchanges);
+
+/*
+ * Exchange-only probe: `rounds` rounds of the kernels' cross-workgroup step
+ * (candidate staged in LDS, published to the exchange area, every
+ * workgroup's words polled, winner handed to every wave) among `grid`
+ * workgroups of 1024 threads, with candidates of `words` 64-bit words (5: a
+ * 3-D pick; 10: a paired kernel's two), and no point work.  Returns the
+ * average microseconds per round: the floor of one FPS pass at that grid.
+ * Measurement aid; no counterpart in the reference.
+ */
+int fpsx_probe_exchange(unsigned grid, unsigned words, unsigned rounds, void *stream,
+                        double *us_per_round);
 
 /*
  * Kernel timing on the current device: fpsx_timing(1) makes every later FPS
@@ -122,9 +166,10 @@ int fpsx_fps_sampling(const float *points, size_t n_points, size_t dim, int layo
  *   d_out     device array of n_samples uint64 indices
  *   tie       FPSX_TIE_HIGHEST or FPSX_TIE_LOWEST
  *   init      initial min-distance (+inf for fps_sampling, FLT_MAX for bucket FPS)
- * The call returns once the work is enqueued; the library's scratch buffers
- * stay in use until the stream reaches that point, so concurrent calls for
- * one device must use the same stream.
+ * The call returns once the work is enqueued.  The library's per-device
+ * scratch buffers stay in use until the stream reaches that point; a later
+ * call on another stream waits for them on the device (an event), so calls
+ * on different streams are correct but do not overlap.
  */
 int fpsx_fps_sampling_device(const float *d_points, size_t ld, size_t n_points, size_t dim,
                              size_t n_samples, const uint64_t *d_start, size_t n_start,

@@ -24,6 +24,9 @@ class CApi:
     def __init__(self, path: str = LIB):
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} is not built (python fpsample_amd/build.py)")
+        from fpsample_amd.device import hip
+
+        hip()  # one HIP runtime for this library and a torch imported later
         L = ctypes.CDLL(path)
         L.fpsx_last_error.restype = ctypes.c_char_p
         L.fpsx_version.restype = ctypes.c_char_p

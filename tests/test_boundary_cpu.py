@@ -7,6 +7,7 @@ CPU.  Reference behaviour followed: src/fpsample/__init__.py:19-65, 174-206
 (C return codes).
 """
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -14,6 +15,8 @@ import pytest
 import fpsample_amd as F
 from oracle import oracle as O
 from tests.capi import CApi, header_functions
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_library_exports_every_header_function(capi):
@@ -27,10 +30,44 @@ def test_version(capi):
     assert capi.L.fpsx_version().decode() == F.__version__
 
 
-def test_bucket_signature_matches_reference_abi(capi):
-    # int bucket_fps_kdline(const float*, size_t, size_t, size_t, size_t, size_t, size_t*)
-    fn = capi.L.bucket_fps_kdline
-    assert fn.restype is ctypes.c_int and len(fn.argtypes) == 7
+def _compile_abi_check(*extra):
+    import subprocess
+
+    src = os.path.join(ROOT, "tests", "abi_signature_check.cpp")
+    cmd = ["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), *extra, src]
+    return subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
+def test_bucket_signature_matches_reference_abi():
+    # include/fpsx.h's declaration has the type of src/wrapper.hpp:117-119 (a
+    # static_assert on the function pointer type, checked by the compiler).
+    p = _compile_abi_check()
+    assert p.returncode == 0, p.stdout
+
+
+def test_bucket_signature_agrees_with_reference_header():
+    # With the reference's own header in the same translation unit, a
+    # different C signature would be a conflicting declaration.
+    ref_src = "/root/reference/src"
+    if not os.path.exists(os.path.join(ref_src, "wrapper.hpp")):
+        pytest.skip("reference sources not present")
+    p = _compile_abi_check("-DWITH_REFERENCE_HEADER", "-I", ref_src, "-w")
+    assert p.returncode == 0, p.stdout
+
+
+def test_abi_check_catches_a_changed_signature(tmp_path):
+    # The check itself is live: a header whose declaration differs fails it.
+    hdr = open(os.path.join(ROOT, "include", "fpsx.h")).read()
+    bad = hdr.replace("size_t start_idx, size_t height, size_t *sampled_point_indices);",
+                      "size_t start_idx, int height, size_t *sampled_point_indices);")
+    assert bad != hdr
+    (tmp_path / "fpsx.h").write_text(bad)
+    import subprocess
+
+    src = os.path.join(ROOT, "tests", "abi_signature_check.cpp")
+    p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", str(tmp_path), src],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert p.returncode != 0
 
 
 def test_capi_validation_codes(capi):
@@ -259,3 +296,36 @@ def test_kdline_permutation_special_values_fuzz(capi):
     big[::1000] = np.nan
     for h in (5, 11):  # large enough for the threaded top levels
         np.testing.assert_array_equal(capi.kdline_perm(big, h)[0], O.kdline_perm(big, h)[0])
+
+
+# ---------------------------------------------------------------- device-tensor entry
+def test_tensor_entry_refuses_host_tensors_before_any_launch():
+    # A host address handed to the kernels would be a GPU memory fault; the
+    # tensor entry must raise first, without touching the device.
+    from fpsample_amd import device as D
+
+    torch = pytest.importorskip("torch")
+    with pytest.raises(TypeError, match="GPU tensor"):
+        D.fps_sampling_tensor(torch.zeros((16, 3), dtype=torch.float32), 4)
+    # numpy exports DLPack on the CPU device: refused by device type
+    with pytest.raises(TypeError, match="ROCm device memory"):
+        D.fps_sampling_tensor(np.zeros((16, 3), np.float32), 4)
+
+
+def test_tensor_entry_checks_rank_before_strides():
+    from fpsample_amd import device as D
+
+    class OneD:
+        __cuda_array_interface__ = {"shape": (16,), "typestr": "<f4", "data": (4096, False),
+                                    "strides": None, "version": 3}
+
+    class Wrong:
+        __cuda_array_interface__ = {"shape": (16, 3), "typestr": "<f8", "data": (4096, False),
+                                    "strides": None, "version": 3}
+
+    with pytest.raises(ValueError, match="2-D"):
+        D.fps_sampling_tensor(OneD(), 4)
+    with pytest.raises(TypeError, match="float32"):
+        D.fps_sampling_tensor(Wrong(), 4)
+    with pytest.raises(TypeError, match="expected a GPU tensor"):
+        D.fps_sampling_tensor([[0.0, 1.0, 2.0]], 1)
