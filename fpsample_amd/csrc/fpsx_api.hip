@@ -221,27 +221,15 @@ static int plan_stream_pruned(Ctx &c, size_t n, int tie, size_t want_g, Plan &p)
 
 static int plan_resident_shape(Ctx &c, size_t n, const KernelEntry *k, Plan &p,
                                PlanKind kind = PlanKind::Resident) {
-  const size_t per = (size_t)k->block * k->ppt;
-  const size_t g = (n + per - 1) / per;
-  if (g > 1 && g > (size_t)c.max_blocks(k, 0)) return -1;
-  if (g > (size_t)k->block) return -1;  // exchange polls one workgroup per lane
-  if (g > (size_t)kPageWords) return -1;
+  const uint32_t g = resident_grid(n, k, (size_t)c.max_blocks(k, 0));
+  if (!g) return -1;
   p.kind = kind;
   p.k = k;
   p.block = (uint32_t)k->block;
-  p.grid = (uint32_t)g;
-  p.chunk = (uint32_t)per;
+  p.grid = g;
+  p.chunk = (uint32_t)(k->block * k->ppt);
   p.shmem = 0;
   return FPSX_OK;
-}
-
-// Per-pass time model fitted to MI355X measurements (microseconds):
-// reduction chain ~0.55, update 12 VALU ops per point over 64 lanes/clock
-// at 2.4 GHz, exchange ~1.15 + 0.0125 * G (1 M points: 3.39 / 3.78 / 4.38 us
-// measured at G = 64 / 128 / 256; the model keeps that order).
-static double model_us(size_t n, size_t g) {
-  const double per_wg = (double)n / (double)g;
-  return 0.55 + per_wg * 12.0 / (64.0 * 2400.0) + (g > 1 ? 1.15 + 0.0125 * (double)g : 0.0);
 }
 
 static int plan_from_tuned(Ctx &c, size_t n, size_t dim, int tie, Plan &p) {
@@ -337,35 +325,10 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
     }
   }
 
-  // Single workgroup: the smallest resident shape that holds the cloud.
-  static const int kSmall[][2] = {{256, 1}, {256, 2}, {256, 4}, {1024, 1}, {1024, 2},
-                                  {1024, 4}, {1024, 8}, {1024, 16}};
-  if (n <= 16384) {
-    for (auto &sh : kSmall) {
-      if ((size_t)sh[0] * sh[1] < n) continue;
-      const KernelEntry *k = find_resident((int)dim, sh[0], sh[1], tie);
-      if (k && plan_resident_shape(c, n, k, p) == FPSX_OK) return FPSX_OK;
-    }
-  }
-  // Multi-workgroup resident: the feasible shape the time model prefers.
-  size_t cnt;
-  const KernelEntry *t = resident_table(&cnt);
-  double best = 1e30;
-  Plan bp;
-  for (size_t i = 0; i < cnt; ++i) {
-    if (t[i].dim != (int)dim || t[i].tie != tie) continue;
-    Plan q;
-    if (plan_resident_shape(c, n, &t[i], q) != FPSX_OK) continue;
-    const double us = model_us(n, q.grid) + (t[i].block == 1024 ? 0.0 : 0.05);
-    if (us < best) {
-      best = us;
-      bp = q;
-    }
-  }
-  if (bp.k) {
-    p = bp;
+  // Register-resident: one workgroup up to 16 K points, else the feasible
+  // shape the per-pass time model prefers.
+  if (plan_resident_only(n, dim, tie, [&](const KernelEntry *k) { return c.max_blocks(k, 0); }, p))
     return FPSX_OK;
-  }
   // Beyond the register file: 3-D clouds stream only the blocks a pick can
   // change; other dimensions stream everything.
   if (dim == 3 && m >= 64 && !(np && np[0] == '1') && plan_stream_pruned(c, n, tie, 0, p) == FPSX_OK)

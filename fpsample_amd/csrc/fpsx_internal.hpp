@@ -70,6 +70,79 @@ struct Ctx {
   size_t events_used = 0;
 };
 
+// Per-pass time model of the resident kernel fitted to MI355X measurements
+// (microseconds): reduction chain ~0.55, update 12 VALU ops per point over 64
+// lanes/clock at 2.4 GHz, exchange ~1.15 + 0.0125 * G (1 M points: 3.39 /
+// 3.78 / 4.38 us measured at G = 64 / 128 / 256; the model keeps that order).
+inline double resident_model_us(size_t n, size_t g) {
+  const double per_wg = (double)n / (double)g;
+  return 0.55 + per_wg * 12.0 / (64.0 * 2400.0) + (g > 1 ? 1.15 + 0.0125 * (double)g : 0.0);
+}
+
+// Grid of a resident-shaped kernel k (points in registers) for n points, or 0
+// when it cannot hold them: more workgroups than co-resident (max_blocks),
+// than the exchange polls (one per lane of B), or than exchange pages.
+inline uint32_t resident_grid(size_t n, const KernelEntry *k, size_t max_blocks) {
+  const size_t per = (size_t)k->block * k->ppt;
+  const size_t g = (n + per - 1) / per;
+  if (g > 1 && g > max_blocks) return 0;
+  if (g > (size_t)k->block || g > 512 /* kPageWords */) return 0;
+  return (uint32_t)g;
+}
+
+// The register-resident kernel for a cloud, without spatial order or HBM
+// state: up to 16 K points one workgroup with the smallest shape that holds
+// the cloud; beyond, the feasible shape the per-pass model prefers.
+// max_blocks(k) gives kernel k's co-resident workgroup count.  Returns false
+// when no resident shape holds the cloud (beyond the register file).
+template <class MaxBlocks>
+bool plan_resident_only(size_t n, size_t dim, int tie, MaxBlocks max_blocks, Plan &p) {
+  size_t cnt;
+  const KernelEntry *t = resident_table(&cnt);
+  auto take = [&](const KernelEntry *k, uint32_t g) {
+    p.kind = PlanKind::Resident;
+    p.k = k;
+    p.block = (uint32_t)k->block;
+    p.grid = g;
+    p.chunk = (uint32_t)(k->block * k->ppt);
+    p.shmem = 0;
+  };
+  static const int kSmall[][2] = {{256, 1}, {256, 2}, {256, 4}, {1024, 1}, {1024, 2},
+                                  {1024, 4}, {1024, 8}, {1024, 16}};
+  if (n <= 16384) {
+    for (auto &sh : kSmall) {
+      if ((size_t)sh[0] * sh[1] < n) continue;
+      for (size_t i = 0; i < cnt; ++i) {
+        const KernelEntry *k = &t[i];
+        if (k->dim != (int)dim || k->tie != tie || k->block != sh[0] || k->ppt != sh[1]) continue;
+        const uint32_t g = resident_grid(n, k, (size_t)max_blocks(k));
+        if (g) {
+          take(k, g);
+          return true;
+        }
+      }
+    }
+  }
+  double best = 1e30;
+  const KernelEntry *bk = nullptr;
+  uint32_t bg = 0;
+  for (size_t i = 0; i < cnt; ++i) {
+    const KernelEntry *k = &t[i];
+    if (k->dim != (int)dim || k->tie != tie) continue;
+    const uint32_t g = resident_grid(n, k, (size_t)max_blocks(k));
+    if (!g) continue;
+    const double us = resident_model_us(n, g) + (k->block == 1024 ? 0.0 : 0.05);
+    if (us < best) {
+      best = us;
+      bk = k;
+      bg = g;
+    }
+  }
+  if (!bk) return false;
+  take(bk, bg);
+  return true;
+}
+
 int fail(int code, const std::string &msg);
 int get_ctx(Ctx **out);
 // Kernel and grid for a cloud of n points of dimension dim and m samples.

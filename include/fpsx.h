@@ -189,6 +189,35 @@ int fpsx_fps_sampling_device_strided(const float *d_points, size_t stride_point,
                                      float *d_out_points, int tie, float init, void *stream);
 
 /*
+ * Allocation-free form of fpsx_fps_sampling_device_strided, for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  No counterpart in the reference.
+ * The call allocates nothing (no hipMalloc, hipFree or allocating library
+ * routine): its only scratch, the cross-workgroup exchange area and status
+ * word (1 MiB + 256 B), is a fixed array in the library's code object that
+ * every GPU the library runs on holds until the process ends.  It runs the
+ * register-resident kernels only (the spatially ordered and streaming plans
+ * need per-point scratch), so it is slower than the strided entry on large
+ * 3-D clouds, and a cloud beyond the register file (about 4 M 3-D points on
+ * MI355X) is refused with FPSX_ETOOBIG before anything is enqueued.  The
+ * picks are the same as every other entry's.
+ *   d_status  optional device uint32: cleared by the call, then receives the
+ *             kernel's status bits once the stream reaches the end of the
+ *             call (bit 0: an exchange timed out, bit 1: a start index >=
+ *             n_points); NULL keeps them in the library's scratch, unread
+ *   stream    NULL means the device's null stream (not the library's stream)
+ * Every call on one GPU shares that GPU's scratch, which keeps the last call's
+ * contents: calls that overlap on one GPU (other streams, other threads)
+ * race, so the caller orders them.  The exchange wait bound is
+ * FPSX_EXCHANGE_TIMEOUT_S (default 2 s), read on the first call per device.
+ */
+int fpsx_fps_sampling_device_noalloc(const float *d_points, size_t stride_point,
+                                     size_t stride_dim, size_t n_points, size_t dim,
+                                     size_t n_samples, const uint64_t *d_start, size_t n_start,
+                                     uint64_t *d_out, float *d_out_points, uint32_t *d_status,
+                                     int tie, float init, void *stream);
+
+/*
  * FPS with nearest-point-distance-updating (NPDU).
  * Replaces fps_npdu_sampling(points, n_samples, k, start_idx)
  * (src/lib.cpp:272-340): the start updates the min-distance of every point;
