@@ -36,7 +36,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", CSRC, "-I", INCLUDE,
 ]
 
-SOURCES_HIP = ["fps_kernels.hip", "fps_batch.hip", "fps_pruned.hip", "fps_npdu.hip", "fpsx_api.hip",
+SOURCES_HIP = ["fps_kernels.hip", "fps_batch.hip", "fps_batch_pruned.hip", "fps_pruned.hip", "fps_npdu.hip", "fpsx_api.hip",
                "fpsx_batch.hip", "fpsx_server.cpp", "fpsx_spatial.hip", "fpsx_probe.hip", "fpsx_noalloc.hip"]
 SOURCES_HOST = ["kdline.cpp"]
 

@@ -24,8 +24,10 @@ struct BatchJob {
   uint32_t s_end;
   uint32_t team;       // workgroups working on the cloud (1..kMaxTeam)
   uint32_t member_base;// claim slot of the team's first member
-  uint32_t share;      // points per member: ceil(n / team) <= B * PPT
+  uint32_t share;      // points per member: ceil(n / team) <= B * PPT (B * PPT for the pruned kernel)
   uint32_t area;       // exchange area of the team (teams only)
+  uint64_t sorted_off; // pruned kernel: the cloud in spatial order, SoA, at sorted + sorted_off (ld = n)
+  uint64_t perm_off;   // pruned kernel: original index of every sorted position, perm + perm_off
 };
 
 struct BatchParams {
@@ -46,5 +48,28 @@ struct BatchParams {
 };
 
 const KernelEntry *batch_table(size_t *count);
+
+// Batched FPS with the spatial order, per-wave box pruning and paired picks
+// (fps_batch_pp): every job is a team of ceil(n / (B * PPT)) members, member r
+// owning sorted positions [r * B * PPT, (r + 1) * B * PPT).  Chunked jobs save
+// their min-distances by sorted position in state[state_off .. + n) and the
+// last pick's coordinates in state[state_off + n .. + 3).
+constexpr uint32_t kPPReplicas = 4;  // exchange replicas of a team
+// Words between a team area's word arrays.  64 (one word per member); a
+// 4 KiB page per array as the single-cloud kernels use was slower here
+// (largest serving request alone, 256-pass chunks: 1033 against 995 ms).
+constexpr uint32_t kPPPitch = kMaxTeam;
+constexpr size_t kPPTeamWords = 2 * kPPReplicas * 16 * kPPPitch;  // 2 buffers x R x kSlotWords x pitch
+
+struct BatchPPParams {
+  BatchParams b;         // jobs, claims, picks, state, status, exchange areas (kPPTeamWords each)
+  const float *sorted;   // spatially ordered clouds (BatchJob::sorted_off)
+  const uint32_t *perm;  // their original indices (BatchJob::perm_off)
+};
+
+const KernelEntry *batch_pp_table(size_t *count);
+// The pruned batch kernel of a shape: one_slot (one workgroup per member slot,
+// all co-resident) or the claim-loop form; null when not instantiated.
+const KernelEntry *batch_pp_kernel(int block, int ppt, bool one_slot);
 
 }  // namespace fpsx

@@ -251,6 +251,49 @@ __device__ __forceinline__ bool exchange(uint64_t *base, uint32_t R, uint32_t pi
   return ok_all;
 }
 
+// Wave-wide min / max of a float (NaN lanes ignored: v_min/v_max return the
+// other operand), result uniform.
+__device__ __forceinline__ float wave_min_f(float v) {
+  int x = __float_as_int(v);
+  const int id = __float_as_int(INFINITY);
+#define STEP(ctrl, rm)                                                                     \
+  x = __float_as_int(fminf(__int_as_float(x),                                              \
+                           __int_as_float(__builtin_amdgcn_update_dpp(id, x, ctrl, rm, 0xf, false))))
+  STEP(0xb1, 0xf); STEP(0x4e, 0xf); STEP(0x114, 0xf); STEP(0x118, 0xf); STEP(0x142, 0xa);
+  STEP(0x143, 0xc);
+#undef STEP
+  return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+__device__ __forceinline__ float wave_max_f(float v) {
+  int x = __float_as_int(v);
+  const int id = __float_as_int(-INFINITY);
+#define STEP(ctrl, rm)                                                                     \
+  x = __float_as_int(fmaxf(__int_as_float(x),                                              \
+                           __int_as_float(__builtin_amdgcn_update_dpp(id, x, ctrl, rm, 0xf, false))))
+  STEP(0xb1, 0xf); STEP(0x4e, 0xf); STEP(0x114, 0xf); STEP(0x118, 0xf); STEP(0x142, 0xa);
+  STEP(0x143, 0xc);
+#undef STEP
+  return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+
+// Lower bound of the squared distance from p to the box [lo, hi], in the
+// point distance's operand order (src/_ext/KDNode.h:105-118 does the same).
+// The gap along j is max(p - hi, lo - p, 0), written without branches: it is
+// p - hi when p > hi, lo - p when p < lo and 0 otherwise, as in the branchy
+// form, including NaN p (both differences NaN, max returns 0) and an empty
+// box (lo = +inf, hi = -inf: gap +inf).
+template <int D>
+__device__ __forceinline__ float box_bound(const float (&p)[D], const float (&lo)[D],
+                                           const float (&hi)[D]) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const float g = fmaxf(fmaxf(p[j] - hi[j], lo[j] - p[j]), 0.0f);
+    acc = (j == 0) ? g * g : acc + g * g;
+  }
+  return acc;
+}
+
 // Records pick s: lane 0 of the calling wave writes the index and, when the
 // caller asked for them, the coordinates (the fused `pc[idx]` gather).  The
 // coordinates are stored one by one from lane 0: selecting pv[j] by lane

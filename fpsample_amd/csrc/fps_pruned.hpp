@@ -38,5 +38,8 @@ struct SpatialScratch;
 // and perm.  Returns a FPSX_* code.
 int spatial_order(const float *pts, uint64_t sp, uint64_t ld, uint32_t n, uint32_t block,
                   float *sorted, uint32_t *perm, SpatialScratch &scratch, hipStream_t st);
+// Size the scratch for clouds of up to n_max points (blocks of `block`), so
+// later spatial_order calls up to that size allocate nothing.
+int spatial_reserve(SpatialScratch &scratch, uint32_t n_max, uint32_t block);
 
 }  // namespace fpsx

@@ -42,7 +42,7 @@ static const KernelEntry *find_batch(int dim, int B, int ppt) {
 
 int build_jobs(const JobSpec *specs, size_t n_specs, const KernelEntry *k, uint32_t share_cap,
                std::vector<BatchJob> &jobs, std::vector<uint32_t> &member_job,
-               uint32_t *n_areas) {
+               uint32_t *n_areas, bool aligned) {
   const uint32_t cap = std::min<uint32_t>((uint32_t)(k->block * k->ppt), share_cap ? share_cap : ~0u);
   std::vector<BatchJob> js(n_specs);
   for (size_t c = 0; c < n_specs; ++c) {
@@ -58,12 +58,14 @@ int build_jobs(const JobSpec *specs, size_t n_specs, const KernelEntry *k, uint3
     J.m = sp.m;
     J.s_begin = sp.s_begin;
     J.s_end = sp.s_end;
+    J.sorted_off = sp.sorted_off;
+    J.perm_off = sp.perm_off;
     uint32_t team = (sp.n + cap - 1) / cap;
     if (team < 1) team = 1;
     if (team > kMaxTeam || team > (uint32_t)k->block)
       return fail(FPSX_ETOOBIG, "cloud of " + std::to_string(sp.n) +
                                     " points exceeds the batch kernel's team capacity");
-    J.share = (sp.n + team - 1) / team;
+    J.share = aligned ? cap : (sp.n + team - 1) / team;
     J.team = (sp.n + J.share - 1) / J.share;  // no empty member
   }
   // Teams first (they form at once while the grid is free), then single
@@ -93,13 +95,18 @@ int build_jobs(const JobSpec *specs, size_t n_specs, const KernelEntry *k, uint3
 
 int BatchRun::prepare(Ctx &c, const KernelEntry *kernel, const std::vector<BatchJob> &js,
                       const std::vector<uint32_t> &mj, uint32_t n_areas, const float *d_pts,
-                      uint64_t *d_out, float *d_state) {
+                      uint64_t *d_out, float *d_state, const float *sorted,
+                      const uint32_t *perm, bool one) {
   k = kernel;
   jobs = js;
   member_job = mj;
   members = (uint32_t)mj.size();
+  pruned = sorted != nullptr;
+  one_slot = one;
   int gmax = c.max_blocks(k, 0);
   if (gmax <= 0) return fail(FPSX_EHIP, "batch kernel cannot be resident");
+  if (one_slot && members > (uint32_t)gmax)
+    return fail(FPSX_ETOOBIG, "one-slot batch launch: more members than co-resident workgroups");
   grid = std::min<uint32_t>(members, (uint32_t)gmax);
   uint32_t max_team = 1;
   for (auto &J : jobs) max_team = std::max(max_team, J.team);
@@ -108,7 +115,7 @@ int BatchRun::prepare(Ctx &c, const KernelEntry *kernel, const std::vector<Batch
   HIP_TRY(d_member_job.reserve(members * sizeof(uint32_t)));
   counter_words = (1 + jobs.size() + 1) / 2 + 1;  // claim + per-job ready counters, in u64 words
   HIP_TRY(d_counters.reserve(counter_words * sizeof(uint64_t)));
-  xchg_words = std::max<size_t>(1, (size_t)n_areas * kTeamXchgWords);
+  xchg_words = std::max<size_t>(1, (size_t)n_areas * (pruned ? kPPTeamWords : kTeamXchgWords));
   HIP_TRY(d_xchg.reserve(xchg_words * sizeof(uint64_t)));
   HIP_TRY(d_status.reserve(sizeof(uint64_t)));
   HIP_TRY(hipMemcpy(d_jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob),
@@ -133,6 +140,9 @@ int BatchRun::prepare(Ctx &c, const KernelEntry *kernel, const std::vector<Batch
   // (the one-pick team loop; same picks).
   const char *np = std::getenv("FPSX_BATCH_NO_PAIR");
   params.pair = (np && np[0] == '1') ? 0u : 1u;
+  pp_params.b = params;
+  pp_params.sorted = sorted;
+  pp_params.perm = perm;
   return FPSX_OK;
 }
 
@@ -140,7 +150,7 @@ int BatchRun::enqueue(hipStream_t st) {
   HIP_TRY(launch_fill_u64((uint64_t *)d_counters.p, counter_words, 0, st));
   HIP_TRY(launch_fill_u64((uint64_t *)d_xchg.p, xchg_words, 0, st));
   HIP_TRY(launch_fill_u64((uint64_t *)d_status.p, 1, 0, st));
-  void *args[] = {&params};
+  void *args[] = {pruned ? (void *)&pp_params : (void *)&params};
   HIP_TRY(hipLaunchKernel(k->fn, dim3(grid), dim3(k->block), args, 0, st));
   return FPSX_OK;
 }

@@ -164,6 +164,8 @@ int status_code(uint32_t st);
 int scratch_acquire(Ctx &c, hipStream_t st);
 int scratch_release(Ctx &c, hipStream_t st);
 struct SpatialScratch *new_spatial_scratch();
+// Frees the scratch's device buffers (owners with a shorter life than the process).
+void delete_spatial_scratch(struct SpatialScratch *s);
 int fps_host(const float *points, size_t n, size_t dim, int layout, size_t m,
              const uint64_t *starts, size_t n_starts, uint64_t *out, int tie, float init);
 

@@ -36,6 +36,7 @@
 #include "fpsx.h"
 #include "fpsx_batch_internal.hpp"
 #include "fpsx_internal.hpp"
+#include "fps_pruned.hpp"
 
 #define FPSX_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -100,6 +101,7 @@ struct Request {
   uint64_t start = 0;
   // device placement (byte offsets into the arena)
   size_t off = 0, bytes = 0, pts_off = 0, state_off = 0, out_off = 0;
+  size_t sorted_off = 0, perm_off = 0;  // pruned kernel: the spatial order
   uint32_t team = 1;
   uint32_t next = 0;  // next pick to compute (0 = not started)
   hipEvent_t uploaded = nullptr;
@@ -118,6 +120,11 @@ using namespace fpsx;
 struct fpsx_server {
   Ctx *ctx = nullptr;
   const KernelEntry *k = nullptr;
+  // The pruned paired kernel (fps_batch_pp, one slot per member): every
+  // request is put in spatial order on the copy stream after its upload.
+  // FPSX_SERVE_PLAIN=1 at creation keeps the plain batch kernel.
+  bool pruned = true;
+  SpatialScratch *spatial = nullptr;
   uint32_t chunk = 256;
   size_t budget = 0;
   void *arena_base = nullptr;
@@ -146,9 +153,13 @@ struct fpsx_server {
     const uint32_t t = (n + cap - 1) / cap;
     return std::max(1u, t);
   }
+  // Points, min-distances (+ the last pick's coordinates), picks; for the
+  // pruned kernel also the spatially ordered points and their original indices.
   size_t footprint(const Request &r) const {
-    return align_up((size_t)r.n * r.dim * sizeof(float)) + align_up((size_t)r.n * sizeof(float)) +
-           align_up((size_t)r.m * sizeof(uint64_t));
+    size_t b = align_up((size_t)r.n * r.dim * sizeof(float)) +
+               align_up(((size_t)r.n + 4) * sizeof(float)) + align_up((size_t)r.m * sizeof(uint64_t));
+    if (pruned) b += align_up((size_t)r.n * r.dim * sizeof(float)) + align_up((size_t)r.n * sizeof(uint32_t));
+    return b;
   }
   void loop();
   int step();  // one admit / launch / retire round; caller does not hold mu
@@ -168,7 +179,9 @@ int fpsx_server::step() {
       r->off = off;
       r->pts_off = off;
       r->state_off = off + align_up((size_t)r->n * r->dim * sizeof(float));
-      r->out_off = r->state_off + align_up((size_t)r->n * sizeof(float));
+      r->out_off = r->state_off + align_up(((size_t)r->n + 4) * sizeof(float));
+      r->sorted_off = r->out_off + align_up((size_t)r->m * sizeof(uint64_t));
+      r->perm_off = r->sorted_off + align_up((size_t)r->n * r->dim * sizeof(float));
       r->t_admit = Clock::now();
       members += r->team;
       admitted.push_back(r);
@@ -181,6 +194,12 @@ int fpsx_server::step() {
     char *base = (char *)arena_base;
     hipError_t e = hipMemcpyAsync(base + r->pts_off, r->soa.data(), r->soa.size() * sizeof(float),
                                   hipMemcpyHostToDevice, copy);
+    if (e == hipSuccess && pruned) {
+      const int rc = spatial_order((const float *)(base + r->pts_off), 1, r->n, r->n,
+                                   (uint32_t)(64 * k->ppt), (float *)(base + r->sorted_off),
+                                   (uint32_t *)(base + r->perm_off), *spatial, copy);
+      if (rc) return rc;
+    }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventRecord(r->uploaded, copy);
     if (e != hipSuccess) return fail(FPSX_EHIP, std::string("upload: ") + hipGetErrorString(e));
@@ -207,6 +226,8 @@ int fpsx_server::step() {
       s.m = r->m;
       s.s_begin = r->next;
       s.s_end = std::min<uint32_t>(r->m, (r->next == 0 ? 1u : r->next) + chunk);
+      s.sorted_off = r->sorted_off / sizeof(float);
+      s.perm_off = r->perm_off / sizeof(uint32_t);
       specs.push_back(s);
       in_launch.push_back(r);
     }
@@ -221,13 +242,16 @@ int fpsx_server::step() {
   std::vector<BatchJob> jobs;
   std::vector<uint32_t> mj;
   uint32_t areas = 0;
-  int rc = build_jobs(specs.data(), specs.size(), k, 0, jobs, mj, &areas);
+  int rc = build_jobs(specs.data(), specs.size(), k, 0, jobs, mj, &areas, pruned);
   if (rc) return rc;
-  // pt/out/state offsets are element offsets from the arena base for all three views
+  // pt/out/state/sorted/perm offsets are element offsets from the arena base for every view
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    rc = run.prepare(*ctx, k, jobs, mj, areas, (const float *)arena_base, (uint64_t *)arena_base,
-                     (float *)arena_base);
+    rc = pruned ? run.prepare(*ctx, k, jobs, mj, areas, (const float *)arena_base,
+                              (uint64_t *)arena_base, (float *)arena_base,
+                              (const float *)arena_base, (const uint32_t *)arena_base, true)
+                : run.prepare(*ctx, k, jobs, mj, areas, (const float *)arena_base,
+                              (uint64_t *)arena_base, (float *)arena_base);
   }
   if (rc) return rc;
   const auto t_l0 = Clock::now();
@@ -311,11 +335,19 @@ FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passe
   std::unique_ptr<fpsx_server> s(new fpsx_server());
   int rc = get_ctx(&s->ctx);
   if (rc) return rc;
-  size_t cnt;
-  const KernelEntry *t = batch_table(&cnt);
   const int B = block ? block : 1024, P = ppt ? ppt : 16;
-  for (size_t i = 0; i < cnt; ++i)
-    if (t[i].dim == 3 && t[i].block == B && t[i].ppt == P) s->k = &t[i];
+  const char *plain = std::getenv("FPSX_SERVE_PLAIN");
+  s->pruned = !(plain && plain[0] == '1');
+  if (s->pruned) {
+    s->k = batch_pp_kernel(B, P, true);
+    if (!s->k) s->pruned = false;  // no pruned kernel of that shape: the plain one
+  }
+  if (!s->pruned) {
+    size_t cnt;
+    const KernelEntry *t = batch_table(&cnt);
+    for (size_t i = 0; i < cnt; ++i)
+      if (t[i].dim == 3 && t[i].block == B && t[i].ppt == P) s->k = &t[i];
+  }
   if (!s->k) return fail(FPSX_EINVAL, "no batch kernel with that shape");
   {
     std::lock_guard<std::mutex> lk(s->ctx->mu);
@@ -327,6 +359,18 @@ FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passe
   hipError_t e = hipMalloc(&s->arena_base, s->budget);
   if (e != hipSuccess) return fail(FPSX_ENOMEM, "arena: " + std::string(hipGetErrorString(e)));
   s->arena.reset(new Arena(s->budget));
+  if (s->pruned) {
+    // Scratch of the spatial order sized once for the largest request a team
+    // can hold, so admissions allocate nothing.
+    s->spatial = new_spatial_scratch();
+    rc = spatial_reserve(*s->spatial, std::min<uint32_t>(s->grid_cap, kMaxTeam) * (uint32_t)(B * P),
+                         (uint32_t)(64 * P));
+    if (rc) {
+      delete_spatial_scratch(s->spatial);
+      hipFree(s->arena_base);
+      return rc;
+    }
+  }
   e = hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -423,6 +467,7 @@ FPSX_EXPORT void fpsx_server_destroy(fpsx_server *s) {
   s->run.d_xchg.free();
   s->run.d_status.free();
   if (s->arena_base) hipFree(s->arena_base);
+  delete_spatial_scratch(s->spatial);
   hipStreamDestroy(s->compute);
   hipStreamDestroy(s->copy);
   delete s;

@@ -29,6 +29,14 @@ struct SpatialScratch {
 
 SpatialScratch *new_spatial_scratch() { return new SpatialScratch(); }
 
+void delete_spatial_scratch(SpatialScratch *S) {
+  if (!S) return;
+  for (DevBuf *b : {&S->part, &S->box, &S->keys, &S->vals, &S->keys_alt, &S->vals_alt, &S->keys64,
+                    &S->keys64_alt, &S->tmp})
+    b->free();
+  delete S;
+}
+
 namespace {
 
 constexpr int kBoxThreads = 256;
@@ -147,6 +155,32 @@ __global__ void gather_sorted(const float *__restrict__ pts, uint64_t sp, uint64
       return fail(_e == hipErrorOutOfMemory ? FPSX_ENOMEM : FPSX_EHIP,               \
                   std::string(#expr) + ": " + hipGetErrorString(_e));                \
   } while (0)
+
+int spatial_reserve(SpatialScratch &S, uint32_t n, uint32_t block) {
+  if (n == 0 || block == 0) return fail(FPSX_EINVAL, "spatial_reserve: empty input");
+  HIP_OR_FAIL(S.part.reserve(kBoxBlocks * 6 * sizeof(float)));
+  HIP_OR_FAIL(S.box.reserve(6 * sizeof(float)));
+  HIP_OR_FAIL(S.keys.reserve((size_t)n * 4));
+  HIP_OR_FAIL(S.vals.reserve((size_t)n * 4));
+  HIP_OR_FAIL(S.keys_alt.reserve((size_t)n * 4));
+  HIP_OR_FAIL(S.vals_alt.reserve((size_t)n * 4));
+  HIP_OR_FAIL(S.keys64.reserve((size_t)n * 8));
+  HIP_OR_FAIL(S.keys64_alt.reserve((size_t)n * 8));
+  // Temporary storage of the two sorts at the largest size (the smallest
+  // block gives the most block-key bits).
+  int hi_bit = 32;
+  const uint32_t nblk = (n + block - 1) / block;
+  while (hi_bit < 64 && (((uint64_t)1 << (hi_bit - 32)) < (uint64_t)nblk)) ++hi_bit;
+  size_t t1 = 0, t2 = 0;
+  HIP_OR_FAIL(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, (uint32_t *)S.keys.p,
+                                                 (uint32_t *)S.keys_alt.p, (uint32_t *)S.vals.p,
+                                                 (uint32_t *)S.vals_alt.p, (int)n, 0, 30));
+  HIP_OR_FAIL(hipcub::DeviceRadixSort::SortKeys(nullptr, t2, (uint64_t *)S.keys64.p,
+                                                (uint64_t *)S.keys64_alt.p, (int)n, 0, hi_bit));
+  // With headroom; spatial_order still grows it should some size ask for more.
+  HIP_OR_FAIL(S.tmp.reserve(std::max(t1, t2) * 2));
+  return FPSX_OK;
+}
 
 int spatial_order(const float *pts, uint64_t sp, uint64_t ld, uint32_t n, uint32_t block,
                   float *sorted, uint32_t *perm, SpatialScratch &S, hipStream_t st) {

@@ -49,8 +49,9 @@ def test_serve_tight_budget_queues(gpu):
     from fpsample_amd.serve import BudgetError, Server
 
     reqs = requests(8, 10, 20000, 60000)
-    # room for about one request at a time: the rest wait for memory
-    with Server(hbm_budget_bytes=2 << 20, chunk_passes=64) as srv:
+    # room for about one request at a time (at most 60 K points: 32 B per
+    # point with the spatial order, 8 B per pick): the rest wait for memory
+    with Server(hbm_budget_bytes=3 << 20, chunk_passes=64) as srv:
         ids = [srv.submit(pc, m, s) for pc, m, s in reqs]
         outs = [srv.wait(r)[0] for r in ids]
         with pytest.raises(BudgetError):
@@ -87,3 +88,43 @@ def test_serve_short_chunks_paired_teams(gpu):
         outs = [srv.wait(r)[0] for r in ids]
     for i, ((pc, m, s), got) in enumerate(zip(reqs, outs)):
         np.testing.assert_array_equal(got, O.fps(pc, min(m, 700), s), err_msg=f"request {i}")
+
+
+@pytest.mark.parametrize("plain", [False, True], ids=["pruned", "plain"])
+def test_serve_default_shape_short_chunks(gpu, monkeypatch, plain):
+    # The default 1024 x 16 shape: the pruned paired kernel (spatial order per
+    # request, teams of ceil(n / 16384), one workgroup per member) and, with
+    # FPSX_SERVE_PLAIN=1, the plain batch kernel.  Chunks of 5 passes end on
+    # both sides of paired rounds; multi-start-free requests of 300 to 70 K
+    # points, ties included.
+    from fpsample_amd.serve import Server
+
+    if plain:
+        monkeypatch.setenv("FPSX_SERVE_PLAIN", "1")
+    else:
+        monkeypatch.delenv("FPSX_SERVE_PLAIN", raising=False)
+    reqs = requests(11, 10, 300, 70000)
+    with Server(hbm_budget_bytes=256 << 20, chunk_passes=5) as srv:
+        ids = [srv.submit(pc, min(m, 400), s) for pc, m, s in reqs]
+        outs = [srv.wait(r)[0] for r in ids]
+    for i, ((pc, m, s), got) in enumerate(zip(reqs, outs)):
+        np.testing.assert_array_equal(got, O.fps(pc, min(m, 400), s), err_msg=f"request {i} n={len(pc)}")
+
+
+def test_serve_pruned_whole_requests_against_oracle(gpu):
+    # Long chunks (whole requests in one launch) and full-length picks on
+    # clustered and lattice clouds of one to five members.
+    from fpsample_amd.serve import Server
+
+    rng = np.random.default_rng(12)
+    clouds = []
+    for n in (5000, 16384, 16385, 40000, 81920):
+        centers = rng.random((8, 3))
+        clouds.append((centers[rng.integers(0, 8, n)] + 0.01 * rng.standard_normal((n, 3))).astype(np.float32))
+    clouds.append(rng.integers(0, 12, (30000, 3)).astype(np.float32))
+    ms = [len(c) // 4 for c in clouds]
+    with Server(hbm_budget_bytes=256 << 20, chunk_passes=1 << 20) as srv:
+        ids = [srv.submit(pc, m, 3) for pc, m in zip(clouds, ms)]
+        outs = [srv.wait(r)[0] for r in ids]
+    for i, (pc, m, got) in enumerate(zip(clouds, ms, outs)):
+        np.testing.assert_array_equal(got, O.fps(pc, m, 3), err_msg=f"cloud {i} n={len(pc)}")
