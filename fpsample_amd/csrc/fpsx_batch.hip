@@ -17,6 +17,7 @@
 #include "fpsx.h"
 #include "fpsx_batch_internal.hpp"
 #include "fpsx_internal.hpp"
+#include "fps_pruned.hpp"
 
 #define FPSX_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -167,6 +168,13 @@ struct fpsx_batch {
   int dim = 0;
   size_t n_clouds = 0;
   BatchRun run;
+  // Pruned kernel (fps_batch_pp): every cloud in spatial order, cloud c's
+  // coordinates at sorted + 3 * offsets[c] (SoA, ld = its size) and original
+  // indices at perm + offsets[c]; blocks of `order_block` positions.
+  DevBuf sorted, perm;
+  SpatialScratch *spatial = nullptr;
+  uint32_t order_block = 0;
+  std::vector<uint64_t> offsets;
   bool fallback = false;  // no batch kernel for this dimension or cloud size: one launch per cloud
   std::vector<JobSpec> specs;
   const float *d_pts = nullptr;
@@ -207,12 +215,48 @@ static int validate_batch(const uint64_t *offsets, size_t n_clouds, const uint64
   return FPSX_OK;
 }
 
-static int batch_prepare_kernel(fpsx_batch *b, const KernelEntry *k) {
+// Put every cloud in spatial order with blocks of `block` positions (once
+// per block size), on the context's stream.  Caller holds the context lock.
+static int batch_order(fpsx_batch *b, uint32_t block) {
+  if (b->order_block == block) return FPSX_OK;
+  const uint64_t total = b->offsets.back();
+  uint32_t nmax = 0;
+  for (auto &s : b->specs) nmax = std::max(nmax, s.n);
+  HIP_TRY(b->sorted.reserve(total * 3 * sizeof(float)));
+  HIP_TRY(b->perm.reserve(total * sizeof(uint32_t)));
+  if (!b->spatial) b->spatial = new_spatial_scratch();
+  int rc = spatial_reserve(*b->spatial, nmax, block);
+  if (rc) return rc;
+  hipStream_t st = b->ctx->stream;
+  for (size_t c = 0; c < b->n_clouds; ++c) {
+    const uint64_t off = b->offsets[c];
+    rc = spatial_order(b->d_pts + off, 1, b->ld, b->specs[c].n, block,
+                       (float *)b->sorted.p + 3 * off, (uint32_t *)b->perm.p + off, *b->spatial, st);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  b->order_block = block;
+  return FPSX_OK;
+}
+
+static int batch_prepare_kernel(fpsx_batch *b, const KernelEntry *k, bool pruned) {
   std::vector<BatchJob> jobs;
   std::vector<uint32_t> mj;
   uint32_t areas = 0;
-  int rc = build_jobs(b->specs.data(), b->specs.size(), k, 0, jobs, mj, &areas);
+  if (pruned) {
+    for (size_t c = 0; c < b->n_clouds; ++c) {
+      b->specs[c].sorted_off = 3 * b->offsets[c];
+      b->specs[c].perm_off = b->offsets[c];
+    }
+  }
+  int rc = build_jobs(b->specs.data(), b->specs.size(), k, 0, jobs, mj, &areas, pruned);
   if (rc) return rc;
+  if (pruned) {
+    rc = batch_order(b, (uint32_t)(64 * k->ppt));
+    if (rc) return rc;
+    return b->run.prepare(*b->ctx, k, jobs, mj, areas, b->d_pts, b->d_out, nullptr,
+                          (const float *)b->sorted.p, (const uint32_t *)b->perm.p, false);
+  }
   return b->run.prepare(*b->ctx, k, jobs, mj, areas, b->d_pts, b->d_out, nullptr);
 }
 
@@ -235,31 +279,54 @@ FPSX_EXPORT int fpsx_batch_create(const float *d_points, size_t ld, size_t dim,
   b->d_pts = d_points;
   b->ld = ld;
   b->d_out = d_out;
+  b->offsets.assign(offsets, offsets + n_clouds + 1);
   Ctx &c = *b->ctx;
   std::lock_guard<std::mutex> lk(c.mu);
 
+  // Candidates: 3-D batches first the pruned paired kernels (spatial order,
+  // wave boxes, two picks per team exchange; FPSX_BATCH_PLAIN=1 leaves them
+  // out), then the plain batch kernels of the dimension.
+  struct Cand {
+    const KernelEntry *k;
+    bool pruned;
+  };
+  std::vector<Cand> cands;
+  const char *plain = std::getenv("FPSX_BATCH_PLAIN");
+  if (dim == 3 && !(plain && plain[0] == '1')) {
+    for (int ppt : {8, 16, 4}) {
+      const KernelEntry *k = batch_pp_kernel(1024, ppt, false);
+      if (k) cands.push_back({k, true});
+    }
+  }
   size_t cnt;
   const KernelEntry *t = batch_table(&cnt);
-  std::vector<const KernelEntry *> cands;
   for (size_t i = 0; i < cnt; ++i)
-    if (t[i].dim == (int)dim) cands.push_back(&t[i]);
+    if (t[i].dim == (int)dim) cands.push_back({&t[i], false});
   if (cands.empty()) {
     b->fallback = true;
     *out = b.release();
     return FPSX_OK;
   }
   const KernelEntry *chosen = nullptr;
+  bool chosen_pruned = false;
   const char *env = std::getenv("FPSX_BATCH_SHAPE");
   int eb = 0, ep = 0;
-  if (env && std::sscanf(env, "%d:%d", &eb, &ep) == 2) chosen = find_batch((int)dim, eb, ep);
+  if (env && std::sscanf(env, "pp:%d:%d", &eb, &ep) == 2) {
+    chosen = batch_pp_kernel(eb, ep, false);
+    chosen_pruned = chosen != nullptr && dim == 3;
+    if (!chosen_pruned) chosen = nullptr;
+  } else if (env && std::sscanf(env, "%d:%d", &eb, &ep) == 2) {
+    chosen = find_batch((int)dim, eb, ep);
+  }
   if (!chosen && (flags & FPSX_BATCH_AUTOTUNE)) {
     // Time one launch of every shape that can hold the batch; keep the fastest.
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
-    for (const KernelEntry *k : cands) {
-      if (batch_prepare_kernel(b.get(), k) != FPSX_OK) continue;
+    for (const Cand &cd : cands) {
+      const KernelEntry *k = cd.k;
+      if (batch_prepare_kernel(b.get(), k, cd.pruned) != FPSX_OK) continue;
       HIP_TRY(hipEventRecord(e0, c.stream));
       if (b->run.enqueue(c.stream) != FPSX_OK) continue;
       HIP_TRY(hipEventRecord(e1, c.stream));
@@ -270,29 +337,43 @@ FPSX_EXPORT int fpsx_batch_create(const float *d_points, size_t ld, size_t dim,
       if (*c.host_status == 0 && ms < best) {
         best = ms;
         chosen = k;
+        chosen_pruned = cd.pruned;
       }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
   }
   if (!chosen) {
-    // Heuristic: the smallest per-workgroup capacity that still fits every
-    // cloud in a team the grid can hold (short reduction chains for small
-    // clouds), falling back to larger shapes.
-    static const int kOrder[][2] = {{256, 16}, {1024, 16}, {256, 32}, {1024, 8}, {256, 8}, {256, 4}};
+    // Heuristic: the first pruned kernel whose teams hold every cloud; else
+    // the smallest plain per-workgroup capacity that still fits every cloud
+    // in a team the grid can hold (short reduction chains for small clouds),
+    // falling back to larger shapes.
     uint32_t nmax = 0;
     for (auto &s : b->specs) nmax = std::max(nmax, s.n);
+    for (const Cand &cd : cands) {
+      if (!cd.pruned) continue;
+      const uint32_t cap = (uint32_t)(cd.k->block * cd.k->ppt);
+      if ((nmax + cap - 1) / cap <= kMaxTeam) {
+        chosen = cd.k;
+        chosen_pruned = true;
+        break;
+      }
+    }
+    static const int kOrder[][2] = {{256, 16}, {1024, 16}, {256, 32}, {1024, 8}, {256, 8}, {256, 4}};
     for (auto &sh : kOrder) {
+      if (chosen) break;
       const KernelEntry *k = find_batch((int)dim, sh[0], sh[1]);
       if (!k) continue;
       if ((nmax + (uint32_t)(sh[0] * sh[1]) - 1) / (uint32_t)(sh[0] * sh[1]) > 8 && sh[0] * sh[1] < 16384)
         continue;
       chosen = k;
-      break;
     }
-    if (!chosen) chosen = cands[0];
+    if (!chosen) {
+      chosen = cands.back().k;
+      chosen_pruned = cands.back().pruned;
+    }
   }
-  rc = batch_prepare_kernel(b.get(), chosen);
+  rc = batch_prepare_kernel(b.get(), chosen, chosen_pruned);
   if (rc == FPSX_ETOOBIG) {
     // Some cloud needs more workgroups than a team may have: every cloud
     // gets a planned launch of its own (the single-cloud kernels have no
@@ -383,6 +464,9 @@ FPSX_EXPORT void fpsx_batch_destroy(fpsx_batch *b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   if (b->capture_stream) hipStreamDestroy(b->capture_stream);
+  b->sorted.free();
+  b->perm.free();
+  delete_spatial_scratch(b->spatial);
   delete b;
 }
 

@@ -8,12 +8,15 @@
 //               first-fit ranges, no per-request hipMalloc) and their team of
 //               workgroups fits the co-resident grid; their points are copied
 //               on a separate stream, overlapping the running kernel;
-//   launch   -- one fps_batch launch advances every admitted, uploaded request
-//               by up to `chunk` passes (min-distances saved in the arena
-//               between launches), so requests that arrive while others run
-//               join at the next launch;
-//   retire   -- finished requests copy their picks back, free their range and
-//               wake their waiters.
+//   launch   -- one batch-kernel launch advances every admitted, uploaded
+//               request by up to `chunk` passes (min-distances saved in the
+//               arena between launches), so requests that arrive while others
+//               run join at a following launch.  Launches are pipelined: the
+//               next one is prepared and queued on the compute stream while the
+//               current one runs (two launch slots), so the GPU does not wait
+//               for the host between chunks;
+//   retire   -- once a launch completes, the requests it finished copy their
+//               picks back, free their range and wake their waiters.
 // Each request's picks equal fps_sampling on its cloud alone: a job's result
 // never depends on what else shares the launch (exact min and argmax, ties to
 // the highest index), and a chunk boundary only saves and restores state.
@@ -106,6 +109,7 @@ struct Request {
   uint32_t next = 0;  // next pick to compute (0 = not started)
   hipEvent_t uploaded = nullptr;
   bool ready = false, done = false;
+  bool finishing = false;  // its last chunk is queued: in no further launch
   int error = FPSX_OK;
   std::vector<uint64_t> picks;
   Clock::time_point t_submit, t_admit, t_done;
@@ -131,7 +135,17 @@ struct fpsx_server {
   std::unique_ptr<Arena> arena;
   uint32_t grid_cap = 0;       // co-resident workgroups of the batch kernel
   hipStream_t compute = nullptr, copy = nullptr;
-  BatchRun run;
+  // Two launch slots: the launch running on the compute stream and the next
+  // one, prepared and queued behind it.
+  struct Launch {
+    BatchRun run;
+    hipEvent_t t0 = nullptr, t1 = nullptr;  // around the kernel, for busy_s
+    std::vector<std::shared_ptr<Request>> reqs;
+    std::vector<uint32_t> s_end;
+    bool busy = false;
+  };
+  Launch slot[2];
+  int cur = 0;  // slot of the oldest queued launch
 
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -163,14 +177,18 @@ struct fpsx_server {
   }
   void loop();
   int step();  // one admit / launch / retire round; caller does not hold mu
+  int finish(Launch &L);  // wait for a queued launch, check it, retire what it finished
 };
 
 int fpsx_server::step() {
   std::vector<std::shared_ptr<Request>> admitted;
   {
     std::unique_lock<std::mutex> lk(mu);
+    // Launches run one after another, so only the requests still to be
+    // launched count against the co-resident grid.
     uint32_t members = 0;
-    for (auto &r : active) members += r->team;
+    for (auto &r : active)
+      if (!r->finishing) members += r->team;
     while (!pending.empty()) {
       auto &r = pending.front();
       if (members + r->team > grid_cap) break;
@@ -204,14 +222,20 @@ int fpsx_server::step() {
     if (e == hipSuccess) e = hipEventRecord(r->uploaded, copy);
     if (e != hipSuccess) return fail(FPSX_EHIP, std::string("upload: ") + hipGetErrorString(e));
   }
-  // Jobs of this launch: every uploaded request, next chunk of passes.
+  // Jobs of the next launch: every uploaded request not yet in its last
+  // chunk, from where the queued launches leave it.
   std::vector<std::shared_ptr<Request>> in_launch;
   std::vector<JobSpec> specs;
+  bool waiting_upload = false;
   {
     std::lock_guard<std::mutex> lk(mu);
     for (auto &r : active) {
+      if (r->finishing) continue;
       if (!r->ready) {
-        if (hipEventQuery(r->uploaded) != hipSuccess) continue;
+        if (hipEventQuery(r->uploaded) != hipSuccess) {
+          waiting_upload = true;
+          continue;
+        }
         r->ready = true;
         r->soa.clear();
         r->soa.shrink_to_fit();
@@ -232,13 +256,18 @@ int fpsx_server::step() {
       in_launch.push_back(r);
     }
   }
+  Launch &head = slot[cur];
   if (specs.empty()) {
-    // Everything admitted is still uploading: wait for the copies instead of
-    // spinning.
-    hipError_t e = hipStreamSynchronize(copy);
-    if (e != hipSuccess) return fail(FPSX_EHIP, std::string("upload: ") + hipGetErrorString(e));
+    if (head.busy) return finish(head);
+    if (waiting_upload) {
+      // Everything admitted is still uploading: wait for the copies instead
+      // of spinning.
+      hipError_t e = hipStreamSynchronize(copy);
+      if (e != hipSuccess) return fail(FPSX_EHIP, std::string("upload: ") + hipGetErrorString(e));
+    }
     return FPSX_OK;
   }
+  Launch &L = head.busy ? slot[cur ^ 1] : head;
   std::vector<BatchJob> jobs;
   std::vector<uint32_t> mj;
   uint32_t areas = 0;
@@ -247,31 +276,49 @@ int fpsx_server::step() {
   // pt/out/state/sorted/perm offsets are element offsets from the arena base for every view
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    rc = pruned ? run.prepare(*ctx, k, jobs, mj, areas, (const float *)arena_base,
-                              (uint64_t *)arena_base, (float *)arena_base,
-                              (const float *)arena_base, (const uint32_t *)arena_base, true)
-                : run.prepare(*ctx, k, jobs, mj, areas, (const float *)arena_base,
-                              (uint64_t *)arena_base, (float *)arena_base);
+    rc = pruned ? L.run.prepare(*ctx, k, jobs, mj, areas, (const float *)arena_base,
+                                (uint64_t *)arena_base, (float *)arena_base,
+                                (const float *)arena_base, (const uint32_t *)arena_base, true)
+                : L.run.prepare(*ctx, k, jobs, mj, areas, (const float *)arena_base,
+                                (uint64_t *)arena_base, (float *)arena_base);
   }
   if (rc) return rc;
-  const auto t_l0 = Clock::now();
-  rc = run.enqueue(compute);
-  if (rc) return rc;
-  hipError_t e = hipStreamSynchronize(compute);
+  hipError_t e = hipEventRecord(L.t0, compute);
   if (e != hipSuccess) return fail(FPSX_EHIP, std::string("serve launch: ") + hipGetErrorString(e));
-  busy_s += secs_since(t_l0, Clock::now());
+  rc = L.run.enqueue(compute);
+  if (rc) return rc;
+  e = hipEventRecord(L.t1, compute);
+  if (e != hipSuccess) return fail(FPSX_EHIP, std::string("serve launch: ") + hipGetErrorString(e));
+  L.reqs = in_launch;
+  L.s_end.resize(specs.size());
+  for (size_t i = 0; i < specs.size(); ++i) {
+    L.s_end[i] = specs[i].s_end;
+    in_launch[i]->next = specs[i].s_end;
+    if (specs[i].s_end >= in_launch[i]->m) in_launch[i]->finishing = true;
+  }
+  L.busy = true;
+  // With a launch queued behind it, wait for the older one and retire.
+  if (&L != &head) return finish(head);
+  return FPSX_OK;
+}
+
+int fpsx_server::finish(Launch &L) {
+  hipError_t e = hipEventSynchronize(L.t1);
+  if (e != hipSuccess) return fail(FPSX_EHIP, std::string("serve launch: ") + hipGetErrorString(e));
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, L.t0, L.t1) == hipSuccess) busy_s += ms * 1e-3;
+  L.busy = false;
+  cur ^= 1;  // the other slot (if queued) is now the oldest
   uint32_t st = 0;
-  e = hipMemcpy(&st, run.d_status.p, sizeof(uint32_t), hipMemcpyDeviceToHost);
+  e = hipMemcpy(&st, L.run.d_status.p, sizeof(uint32_t), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return fail(FPSX_EHIP, std::string("status: ") + hipGetErrorString(e));
   if (st) return status_code(st);
   ++launches;
-  // Retire finished requests.
+  // Retire the requests this launch finished.
   std::vector<std::shared_ptr<Request>> finished;
-  for (size_t i = 0; i < in_launch.size(); ++i) {
-    auto &r = in_launch[i];
-    r->next = specs[i].s_end;
-    if (r->next >= r->m) finished.push_back(r);
-  }
+  for (size_t i = 0; i < L.reqs.size(); ++i)
+    if (L.s_end[i] >= L.reqs[i]->m) finished.push_back(L.reqs[i]);
+  L.reqs.clear();
   for (auto &r : finished) {
     r->picks.resize(r->m);
     e = hipMemcpy(r->picks.data(), (char *)arena_base + r->out_off, r->m * sizeof(uint64_t),
@@ -376,6 +423,19 @@ FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passe
   if (e != hipSuccess) {
     const std::string msg = "streams: " + std::string(hipGetErrorString(e));
     if (s->compute) hipStreamDestroy(s->compute);
+    delete_spatial_scratch(s->spatial);
+    hipFree(s->arena_base);
+    return fail(FPSX_EHIP, msg);
+  }
+  for (auto &L : s->slot) {
+    if (e == hipSuccess) e = hipEventCreate(&L.t0);
+    if (e == hipSuccess) e = hipEventCreate(&L.t1);
+  }
+  if (e != hipSuccess) {
+    const std::string msg = "events: " + std::string(hipGetErrorString(e));
+    hipStreamDestroy(s->compute);
+    hipStreamDestroy(s->copy);
+    delete_spatial_scratch(s->spatial);
     hipFree(s->arena_base);
     return fail(FPSX_EHIP, msg);
   }
@@ -461,11 +521,15 @@ FPSX_EXPORT void fpsx_server_destroy(fpsx_server *s) {
   if (s->thread.joinable()) s->thread.join();
   hipStreamSynchronize(s->compute);
   hipStreamSynchronize(s->copy);
-  s->run.d_jobs.free();
-  s->run.d_member_job.free();
-  s->run.d_counters.free();
-  s->run.d_xchg.free();
-  s->run.d_status.free();
+  for (auto &L : s->slot) {
+    L.run.d_jobs.free();
+    L.run.d_member_job.free();
+    L.run.d_counters.free();
+    L.run.d_xchg.free();
+    L.run.d_status.free();
+    if (L.t0) hipEventDestroy(L.t0);
+    if (L.t1) hipEventDestroy(L.t1);
+  }
   if (s->arena_base) hipFree(s->arena_base);
   delete_spatial_scratch(s->spatial);
   hipStreamDestroy(s->compute);

@@ -149,3 +149,47 @@ def test_batch_paired_teams_match_one_pick(gpu, monkeypatch):
     for i, (p, o, w) in enumerate(zip(paired, one, oracle_all(clouds, ms, starts))):
         np.testing.assert_array_equal(o, w, err_msg=f"one-pick cloud {i} n={len(clouds[i])}")
         np.testing.assert_array_equal(p, w, err_msg=f"paired cloud {i} n={len(clouds[i])}")
+
+
+@pytest.mark.parametrize("shape", ["pp:1024:8", "pp:1024:16", "pp:1024:4"])
+def test_batch_pruned_kernels(gpu, monkeypatch, shape):
+    # The pruned paired batch kernels (spatial order per cloud, wave boxes,
+    # two picks per team exchange, work-queue claims): clouds of 50..70 K
+    # points form teams of 1..18 on these shapes; uniform, sphere, integer
+    # ties, all points equal and special values; random starts.
+    import fpsample_amd as F
+    from fpsample_amd import batch as Bt
+
+    monkeypatch.setenv("FPSX_BATCH_SHAPE", shape)
+    rng = np.random.default_rng(41)
+    clouds, ms, starts = [], [], []
+    for i in range(30):
+        n = int(np.exp(rng.uniform(np.log(50), np.log(70000))))
+        kind = i % 5
+        if kind == 0:
+            pc = rng.random((n, 3))
+        elif kind == 1:
+            v = rng.standard_normal((n, 3))
+            pc = v / np.linalg.norm(v, axis=1, keepdims=True)
+        elif kind == 2:
+            pc = rng.integers(0, 5, (n, 3))
+        elif kind == 3:
+            pc = np.tile(rng.random((1, 3)), (n, 1))
+        else:
+            pc = rng.random((n, 3)) * 4e19
+            pc[rng.integers(0, n, 2)] = np.nan
+        clouds.append(pc.astype(np.float32))
+        ms.append(int(rng.integers(1, min(n, 2500) + 1)))
+        starts.append(int(rng.integers(0, n)))
+    log(f"{shape}: sizes {[len(c) for c in clouds]}")
+    want = oracle_all(clouds, ms, starts)
+    got = F.fps_sampling_batch(clouds, ms, starts)
+    for i, (g, w) in enumerate(zip(got, want)):
+        np.testing.assert_array_equal(g, w, err_msg=f"{shape} cloud {i} n={len(clouds[i])}")
+    b = Bt.DeviceBatch(clouds, ms, starts, graph=True)
+    assert b.info()["ppt"] == int(shape.split(":")[2])
+    for rep in range(3):
+        b.run()
+        b.status()
+        np.testing.assert_array_equal(b.picks_flat(), np.concatenate(want), err_msg=f"replay {rep}")
+    b.close()
