@@ -635,6 +635,508 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   if (recorder && lane == 0) f.status[1] = e;
 }
 
+// Paired form beyond 64 workgroups: PW waves poll the exchange, 64
+// workgroups each (G <= 64 * PW).  Each evaluates its workgroups' pairs and
+// the waves combine partial results through LDS in three steps (the best A;
+// the best pick known after p1; whether any workgroup whose A moved could
+// beat it) -- two LDS barriers more per round than fps_pruned_pair, whose
+// single polling wave evaluates the round alone.  Otherwise the same kernel.
+template <int D, int B, int PPT, int TIE, int PW>
+__global__ void __launch_bounds__(B) fps_pruned_pair_wide(PrunedParams a) {
+  constexpr int W = B / 64;
+  constexpr int NW = 2 + D;      // value, original index, coordinates
+  constexpr int NP = 2 * NW;     // words published per workgroup: A, then A'
+  constexpr int PER_WG = B * PPT;
+  static_assert(PW >= 2 && PW <= W, "polling waves");
+  __shared__ int s_top[2][NW + 1][W];  // wave bests + "changed" flags, by exchange parity
+  __shared__ int s_la[2][NW][W];       // wave lookahead bests
+  __shared__ int s_mu[2][PW][2];       // M of the round of this parity (partial per polling wave)
+  __shared__ float s_gbox[2 * D][64 * PW];  // every workgroup's bounding box
+  __shared__ float s_wbox[2 * D][W];   // this workgroup's wave boxes
+  __shared__ int s_okw[PW];            // box exchange: every workgroup arrived, per polling wave
+  // The polling waves' partial results of a round.
+  __shared__ int s_pa[2][PW][NW + 1];  // best A: words, then its workgroup
+  __shared__ int s_pb[2][PW][NW + 2];  // best known pick after p1 (value, index, coordinates),
+                                       // then the best A that moved under p1 (value, index)
+  __shared__ int s_pf[2][PW];          // every candidate arrived
+  __shared__ uint32_t s_perm[PER_WG];
+
+  const FpsParams &f = a.fps;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  // Wave-uniform in a scalar register: LDS addresses indexed by the wave are
+  // then scalar too, rather than vector values kept (and spilled) across the loop.
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t G = gridDim.x;
+  const uint32_t g = blockIdx.x;
+  const uint32_t base = g * (uint32_t)PER_WG;
+  const uint32_t wbase = base + (uint32_t)(wave * 64 * PPT);
+  const bool recorder = (g == 0 && tid >= B - 64);
+
+  for (uint32_t q = tid; q < PER_WG; q += B)
+    s_perm[q] = (base + q < f.n) ? a.perm[base + q] : 0u;
+
+  float c[PPT][D];
+  float dm[PPT];
+  float lo[D], hi[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    lo[j] = INFINITY;
+    hi[j] = -INFINITY;
+  }
+  // Unconditional loads of a clamped position (a load under a branch would
+  // wait for each one in turn); padding slots are then overwritten.
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint32_t i = wbase + (uint32_t)(k * 64 + lane);
+    const bool live = i < f.n;
+    const uint32_t ic = live ? i : f.n - 1u;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const float v = a.sorted[(size_t)j * a.sorted_ld + ic];
+      c[k][j] = live ? v : 0.0f;
+      lo[j] = live ? fminf(lo[j], v) : lo[j];
+      hi[j] = live ? fmaxf(hi[j], v) : hi[j];
+    }
+    dm[k] = live ? f.init : -1.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    lo[j] = wave_min_f(lo[j]);
+    hi[j] = wave_max_f(hi[j]);
+  }
+  lds_barrier();  // s_perm complete
+
+  // Best (value, original index, coordinates) of this wave over val(k).
+  // Each lane carries its best slot's coordinates through the scan (static
+  // indices only: selecting c[slot] afterwards by a uniform slot number is
+  // turned by the compiler into a dynamic index, which puts c in scratch).
+  auto wave_best = [&](auto val, int &ov, int &oi, float(&oc)[D]) {
+    int bv = INT_MIN;
+    int bk = 0;
+    float bc[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) bc[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int b = __float_as_int(val(k));
+      const bool take = (TIE == kTieHighest) ? (b >= bv) : (b > bv);
+      bv = take ? b : bv;
+      bk = take ? k : bk;
+#pragma unroll
+      for (int j = 0; j < D; ++j) bc[j] = take ? c[k][j] : bc[j];
+    }
+    const uint32_t bpos = (uint32_t)(bk * 64 + lane);
+    int wv;
+    const int wl = wave_argmax_lane<TIE>(bv, bpos, wv);
+    const uint32_t wpos = (uint32_t)readlane_i((int)bpos, wl);
+#pragma unroll
+    for (int j = 0; j < D; ++j) oc[j] = readlane_f(bc[j], wl);
+    ov = wv;
+    oi = (int)s_perm[wave * 64 * PPT + wpos];
+  };
+
+  float p1[D], p2[D];
+  uint32_t i1 = load_forced<D>(f, 0, p1);
+#pragma unroll
+  for (int j = 0; j < D; ++j) p2[j] = 0.0f;
+  if (recorder) record_pick<D>(f, 0, i1, p1);
+
+  int cv = INT_MIN;  // this wave's best (uniform)
+  int ci = 0;
+  float cc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) cc[j] = 0.0f;
+  bool cached = false;
+
+  // Forced prefix (multi-start): one pick per pass, no exchange.
+  uint32_t s = 1;
+  for (; s < f.m && s < f.n_starts; ++s) {
+    if (!cached || box_bound<D>(p1, lo, hi) < __int_as_float(cv)) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p1), dm[k]);
+      wave_best([&](int k) { return dm[k]; }, cv, ci, cc);
+      cached = true;
+    }
+    i1 = load_forced<D>(f, s, p1);
+    if (recorder) record_pick<D>(f, s, i1, p1);
+  }
+
+  int ga[NW], gl_[NW];  // the workgroup's A and A' (uniform), kept while unchanged
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    ga[k] = 0;
+    gl_[k] = 0;
+  }
+  bool have2 = false;
+  bool la_valid = false;    // gl_ is the lookahead of the current state
+  int mu_v = INT_MIN;       // M: best A of the certainly-unchanged others
+  uint32_t mu_i = 0;
+  uint32_t e = 0;
+  // Replicas: at most 4 for pairs (measured on gauss1m: 4 replicas 31.7 ms
+  // per launch, 6 replicas 32.4 ms; fewer polls per page do not pay for the
+  // extra publishes).
+  const int K = min(min((int)f.replicas, 64 / NP), 4);
+
+  // Bounding boxes of all workgroups: one exchange (number 1) of 2*D words.
+  if (tid < 2 * PW) {
+    s_mu[tid / PW][tid % PW][0] = INT_MIN;
+    s_mu[tid / PW][tid % PW][1] = 0;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      s_wbox[j][wave] = lo[j];
+      s_wbox[D + j][wave] = hi[j];
+    }
+  }
+  // Several polling waves: wave 0 publishes, wave p polls workgroups
+  // 64p .. 64p + 63.
+  lds_barrier();
+  ++e;
+  uint64_t *bufs = f.xchg + (size_t)(e & 1u) * f.replicas * kSlotWords * kPageWords;
+  if (wave == 0) {
+    float bw[2 * D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      bw[j] = (lane < W) ? s_wbox[j][lane] : INFINITY;
+      bw[D + j] = (lane < W) ? s_wbox[D + j][lane] : -INFINITY;
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      bw[j] = wave_min_f(bw[j]);
+      bw[D + j] = wave_max_f(bw[D + j]);
+    }
+    if (lane < K * 2 * D) {
+      const int rep = lane / (2 * D);
+      const int k = lane - rep * (2 * D);
+      float word = bw[0];
+#pragma unroll
+      for (int q = 1; q < 2 * D; ++q) {
+        float wq = bw[q];
+        asm volatile("" : "+v"(wq));
+        word = (k == q) ? wq : word;
+      }
+      xchg_store(bufs + ((size_t)rep * kSlotWords + k) * kPageWords + g,
+                 ((uint64_t)(uint32_t)__float_as_int(word) << 32) | e);
+    }
+  }
+  if (wave < PW) {
+    const uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * kPageWords;
+    const uint32_t q = (uint32_t)(wave * 64 + lane);
+    bool ok = true;
+    if (q < G) {
+      uint64_t w[2 * D];
+      const uint64_t deadline = now_ticks() + f.spin_ticks;
+      for (;;) {
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < 2 * D; ++k) w[k] = xchg_load(buf + (size_t)k * kPageWords + q);
+#pragma unroll
+        for (int k = 0; k < 2 * D; ++k) all = all && ((uint32_t)w[k] == e);
+        if (all) break;
+        if (now_ticks() > deadline) {
+          ok = false;
+          break;
+        }
+        for (uint32_t z = 0; z < f.backoff; ++z) __builtin_amdgcn_s_sleep(1);
+      }
+#pragma unroll
+      for (int k = 0; k < 2 * D; ++k) s_gbox[k][q] = __int_as_float((int)(w[k] >> 32));
+    }
+    const bool all_ok = __ballot(q < G && !ok) == 0;
+    if (lane == 0) s_okw[wave] = all_ok ? 1 : 0;
+  }
+  lds_barrier();
+  bool all_arrived = true;
+#pragma unroll
+  for (int p = 0; p < PW; ++p) all_arrived = all_arrived && s_okw[p] != 0;
+  if (!all_arrived) {
+    if (tid == 0) atomicOr(f.status, kStatusTimeout);
+    return;
+  }
+
+  while (s < f.m) {
+    // Apply the pending picks.  A wave whose box is at least as far from a
+    // pivot as its largest min-distance cannot change under that pivot.
+    const bool need1 = !cached || box_bound<D>(p1, lo, hi) < __int_as_float(cv);
+    const bool need2 = have2 && box_bound<D>(p2, lo, hi) < __int_as_float(cv);
+    if (need1) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p1), dm[k]);
+    }
+    if (need2) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p2), dm[k]);
+    }
+    if (need1 || need2) {
+      wave_best([&](int k) { return dm[k]; }, cv, ci, cc);
+      cached = true;
+    }
+
+    ++e;
+    const int par = (int)(e & 1u);
+    if (lane == 0) {
+      s_top[par][0][wave] = cv;
+      s_top[par][1][wave] = ci;
+#pragma unroll
+      for (int j = 0; j < D; ++j) s_top[par][2 + j][wave] = __float_as_int(cc[j]);
+      s_top[par][NW][wave] = (need1 || need2) ? 1 : 0;
+    }
+    lds_barrier();
+    int tw[NW + 1];
+#pragma unroll
+    for (int k = 0; k <= NW; ++k) tw[k] = (lane < W) ? s_top[par][k][lane] : 0;
+    if (lane >= W) tw[0] = INT_MIN;
+    // M of the previous round (written by the polling waves before this barrier).
+    const int pv = (lane < PW) ? s_mu[par ^ 1][lane % PW][0] : INT_MIN;
+    const int pi = (lane < PW) ? s_mu[par ^ 1][lane % PW][1] : 0;
+    int mv;
+    const int ml = wave_argmax_lane<TIE>(pv, (uint32_t)pi, mv);
+    mu_v = mv;
+    mu_i = (uint32_t)readlane_i(pi, ml);
+    // The first round computes A everywhere: after a forced prefix a wave's
+    // candidate is cached, but the workgroup's A has never been formed.
+    const bool changed = e == 2u || __ballot(lane < W && tw[NW] != 0) != 0;
+    if (changed) {
+      int gv;
+      const int wl = wave_argmax_lane<TIE>(tw[0], (uint32_t)tw[1], gv);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) ga[k] = readlane_i(tw[k], wl);
+    }
+    const bool needL = (changed || !la_valid) && beats<TIE>(ga[0], (uint32_t)ga[1], mu_v, mu_i);
+    if (changed) la_valid = false;
+    if (needL) {
+      // This workgroup could hold the next pick: its lookahead.  A wave
+      // whose best point keeps its value under A (the update's own
+      // arithmetic) still has that best after the hypothetical update; L =
+      // the best of those.  Only a wave whose best moves and could still beat
+      // L sweeps its points -- normally just the wave holding A.  Every wave
+      // derives the same set from the wave bests in LDS.
+      la_valid = true;
+      float pa[D], tc[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        pa[j] = __int_as_float(ga[2 + j]);
+        tc[j] = __int_as_float(tw[2 + j]);
+      }
+      const float tpost = min_keep(sqdist<D>(tc, pa), __int_as_float(tw[0]));
+      const bool texact = lane < W && __float_as_int(tpost) == tw[0];
+      int lbv;
+      const int lbl = wave_argmax_lane<TIE>(texact ? tw[0] : INT_MIN, (uint32_t)tw[1], lbv);
+      const uint32_t lbi = (uint32_t)readlane_i(tw[1], lbl);
+      const uint64_t sweepers =
+          __ballot(lane < W && !texact && beats<TIE>(tw[0], (uint32_t)tw[1], lbv, lbi));
+      int lv = cv, li = ci;
+      float lc[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) lc[j] = cc[j];
+      if ((sweepers >> wave) & 1ull)
+        wave_best([&](int k) { return min_keep(sqdist<D>(c[k], pa), dm[k]); }, lv, li, lc);
+      if (lane == 0) {
+        s_la[par][0][wave] = lv;
+        s_la[par][1][wave] = li;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s_la[par][2 + j][wave] = __float_as_int(lc[j]);
+      }
+      lds_barrier();
+      int lw[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) lw[k] = (lane < W) ? s_la[par][k][lane] : 0;
+      if (lane >= W) lw[0] = INT_MIN;
+      int lgv;
+      const int wl2 = wave_argmax_lane<TIE>(lw[0], (uint32_t)lw[1], lgv);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) gl_[k] = readlane_i(lw[k], wl2);
+    }
+
+    int sv = INT_MIN, si = 0;  // polling waves: this lane's workgroup's A (value, index)
+    uint32_t i2 = 0;  // the round's second pick (valid when two)
+    bool two = false;
+    // Several polling waves.  Wave 0 publishes; wave p polls workgroups
+    // 64p .. 64p + 63 (lane -> workgroup q) and the waves combine their
+    // partial results through LDS.
+    int x[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) x[k] = 0;
+    const uint32_t q = (uint32_t)(wave * 64 + lane);
+    const bool live = wave < PW && q < G;
+    bool valid = false;
+    uint64_t *bufs = f.xchg + (size_t)(e & 1u) * f.replicas * kSlotWords * kPageWords;
+    if (wave == 0 && lane < K * NP) {
+      const int rep = lane / NP;
+      const int k = lane - rep * NP;
+      int word = 0;
+#pragma unroll
+      for (int qq = 0; qq < NW; ++qq) {
+        int wa = ga[qq], wb = gl_[qq];
+        asm volatile("" : "+v"(wa), "+v"(wb));
+        word = (k == qq) ? wa : word;
+        word = (k == NW + qq) ? wb : word;
+      }
+      uint32_t poff = (uint32_t)(((rep * kSlotWords + k) * kPageWords + (int)g) * sizeof(uint64_t));
+      asm volatile("" : "+v"(poff));
+      xchg_store(reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(bufs) + poff),
+                 ((uint64_t)(uint32_t)word << 32) | e);
+    }
+    if (wave < PW) {
+      // Step 1: poll, then this wave's best A.
+      const uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * kPageWords;
+      uint32_t loff = q * (uint32_t)sizeof(uint64_t);
+      bool ok = true;
+      if (live) {
+        uint64_t w[NP];
+        const uint64_t deadline = now_ticks() + f.spin_ticks;
+        for (;;) {
+          bool all = true;
+          asm volatile("" : "+v"(loff));
+#pragma unroll
+          for (int k = 0; k < NP; ++k)
+            w[k] = xchg_load(reinterpret_cast<const uint64_t *>(
+                reinterpret_cast<const char *>(buf + (size_t)k * kPageWords) + loff));
+#pragma unroll
+          for (int k = 0; k < NP; ++k) all = all && ((uint32_t)w[k] == e);
+          if (all) break;
+          if (now_ticks() > deadline) {
+            ok = false;
+            break;
+          }
+          for (uint32_t z = 0; z < f.backoff; ++z) __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) x[k] = (int)(w[k] >> 32);
+      }
+      valid = live && ok;
+      int v1;
+      const int l1 = wave_argmax_lane<TIE>(valid ? x[0] : INT_MIN, (uint32_t)x[1], v1);
+      int r1[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) r1[k] = readlane_i(x[k], l1);
+      const bool wave_ok = __ballot(live && !ok) == 0;
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) s_pa[par][wave][k] = r1[k];
+        s_pa[par][wave][0] = v1;  // INT_MIN when the wave polled no workgroup
+        s_pa[par][wave][NW] = wave * 64 + l1;
+        s_pf[par][wave] = wave_ok ? 1 : 0;
+      }
+    }
+    lds_barrier();
+    if (wave < PW) {
+      // Step 2: p1 over the partial bests, then each A after p1 (the
+      // update's own arithmetic), this wave's best known pick after p1 and
+      // the best of its A's that moved (an unknown workgroup's best: the
+      // round yields two picks only when no such A beats the known best).
+      // (Lane p holds partial p's words: all loads issued at once, the
+      // winner's words then taken by readlane.)
+      int pa[NW + 1];
+#pragma unroll
+      for (int k = 0; k <= NW; ++k) pa[k] = s_pa[par][lane % PW][k];
+      if (lane >= PW) pa[0] = INT_MIN;
+      int t1;
+      const int lw = wave_argmax_lane<TIE>(pa[0], (uint32_t)pa[1], t1);
+      const uint32_t w1 = (uint32_t)readlane_i(pa[NW], lw);
+      float q1[D], qa[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        q1[j] = __int_as_float(readlane_i(pa[2 + j], lw));
+        qa[j] = __int_as_float(x[2 + j]);
+      }
+      const float post = min_keep(sqdist<D>(qa, q1), __int_as_float(x[0]));
+      const bool moved = valid && q != w1 && __float_as_int(post) != x[0];
+      const bool mine = valid && q == w1;
+      const int tv = mine ? x[NW] : ((valid && !moved) ? x[0] : INT_MIN);
+      const uint32_t ti = (uint32_t)(mine ? x[NW + 1] : x[1]);
+      int v2;
+      const int l2 = wave_argmax_lane<TIE>(tv, ti, v2);
+      const int i2w = readlane_i((int)ti, l2);
+      int r2[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) r2[j] = readlane_i(mine ? x[NW + 2 + j] : x[2 + j], l2);
+      int vm;
+      const int lm = wave_argmax_lane<TIE>(moved ? x[0] : INT_MIN, (uint32_t)x[1], vm);
+      const int im = readlane_i(x[1], lm);
+      if (lane == 0) {
+        s_pb[par][wave][0] = v2;
+        s_pb[par][wave][1] = i2w;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s_pb[par][wave][2 + j] = r2[j];
+        s_pb[par][wave][NW] = vm;
+        s_pb[par][wave][NW + 1] = im;
+      }
+      sv = valid ? x[0] : INT_MIN;
+      si = x[1];
+    }
+    lds_barrier();
+    // Every wave: the round's result from the partials.
+    bool all_ok = true;
+#pragma unroll
+    for (int p = 0; p < PW; ++p) all_ok = all_ok && s_pf[par][p] != 0;
+    if (!all_ok) {
+      if (tid == 0) atomicOr(f.status, kStatusTimeout);
+      return;
+    }
+    {
+      // Lane p holds partial p's words (loads issued at once).
+      int pa[NW], pb[NW + 2];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) pa[k] = s_pa[par][lane % PW][k];
+#pragma unroll
+      for (int k = 0; k < NW + 2; ++k) pb[k] = s_pb[par][lane % PW][k];
+      if (lane >= PW) {
+        pa[0] = INT_MIN;
+        pb[0] = INT_MIN;
+        pb[NW] = INT_MIN;
+      }
+      int t1;
+      const int lw = wave_argmax_lane<TIE>(pa[0], (uint32_t)pa[1], t1);
+      i1 = (uint32_t)readlane_i(pa[1], lw);
+#pragma unroll
+      for (int j = 0; j < D; ++j) p1[j] = __int_as_float(readlane_i(pa[2 + j], lw));
+      int v2;
+      const int lb = wave_argmax_lane<TIE>(pb[0], (uint32_t)pb[1], v2);
+      i2 = (uint32_t)readlane_i(pb[1], lb);
+#pragma unroll
+      for (int j = 0; j < D; ++j) p2[j] = __int_as_float(readlane_i(pb[2 + j], lb));
+      int vmax;
+      const int lmax = wave_argmax_lane<TIE>(pb[NW], (uint32_t)pb[NW + 1], vmax);
+      const uint32_t imax = (uint32_t)readlane_i(pb[NW + 1], lmax);
+      // p2 is the next pick unless some moved A beats it (an unknown
+      // workgroup may then hold a better point).
+      two = vmax == INT_MIN || beats<TIE>(v2, i2, vmax, imax);
+    }
+    have2 = two && s + 1 < f.m;
+    if (wave < PW) {
+      // This wave's part of M (workgroups 64p .. 64p + 63), read after the
+      // next round's first barrier.
+      float bl[D], bh[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        bl[j] = s_gbox[j][q];
+        bh[j] = s_gbox[D + j][q];
+      }
+      const float avf = __int_as_float(sv);
+      const bool keep = sv != INT_MIN && !(box_bound<D>(p1, bl, bh) < avf) &&
+                        !(two && box_bound<D>(p2, bl, bh) < avf);
+      int mv;
+      const int ml = wave_argmax_lane<TIE>((keep && q != g) ? sv : INT_MIN, (uint32_t)si, mv);
+      const int mi = readlane_i(si, ml);
+      if (lane == 0) {
+        s_mu[par][wave][0] = mv;
+        s_mu[par][wave][1] = mi;
+      }
+    }
+    if (recorder) {
+      record_pick<D>(f, s, i1, p1);
+      if (have2) record_pick<D>(f, s + 1, i2, p2);
+    }
+    s += have2 ? 2u : 1u;
+  }
+  // The launch's exchange count (the box exchange included) in the second
+  // status word: how many rounds the pairing saved is data-dependent.
+  if (recorder && lane == 0) f.status[1] = e;
+}
+
 // ---------------------------------------------------------------------------
 // Streaming form for clouds beyond the register file.  The points stay in
 // HBM in the same spatial order, cut into blocks of kSBlock positions (each
@@ -879,9 +1381,24 @@ static const KernelEntry kPair[] = {
 };
 #undef FPSX_PAIR
 
+#define FPSX_PAIR_WIDE(D, B, P, PW)                                                          \
+  {D, B, P, kTieHighest, (const void *)&fps_pruned_pair_wide<D, B, P, kTieHighest, PW>}, \
+  {D, B, P, kTieLowest, (const void *)&fps_pruned_pair_wide<D, B, P, kTieLowest, PW>}
+
+static const KernelEntry kPairWide[] = {
+    FPSX_PAIR_WIDE(3, 1024, 16, 2), FPSX_PAIR_WIDE(3, 1024, 16, 4),
+};
+#undef FPSX_PAIR_WIDE
+
 const KernelEntry *pair_table(size_t *count) {
   *count = sizeof(kPair) / sizeof(kPair[0]);
   return kPair;
+}
+
+const KernelEntry *pair_wide_kernel(int block, int ppt, int tie, int polling_waves) {
+  const int idx = polling_waves == 2 ? 0 : polling_waves == 4 ? 2 : -1;
+  if (idx < 0 || block != 1024 || ppt != 16) return nullptr;
+  return &kPairWide[idx + (tie == kTieLowest ? 1 : 0)];
 }
 
 }  // namespace fpsx

@@ -20,6 +20,10 @@ const KernelEntry *pruned_table(size_t *count);
 // Paired form (same parameters): up to two picks per exchange; 2 <= G <= 64.
 const KernelEntry *pair_table(size_t *count);
 constexpr uint32_t kPairMaxGrid = 64;
+// Paired form with 2 or 4 polling waves (G <= 64 * polling_waves), 1024 x 16
+// only; null for other shapes.
+const KernelEntry *pair_wide_kernel(int block, int ppt, int tie, int polling_waves);
+constexpr uint32_t kPairWideMaxGrid = 256;
 
 // Streaming pruned kernel: blocks of kSBlock positions in spatial order,
 // kSMetaWords LDS words per block (box, cached candidate, list entry).

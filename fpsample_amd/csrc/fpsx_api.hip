@@ -249,13 +249,30 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
       int got = std::sscanf(s.c_str(), "spruned:%d", &g);
       return plan_stream_pruned(c, n, tie, got == 1 ? (size_t)g : 0, p);
     }
+    int pw = 0;
+    if (std::sscanf(s.c_str(), "pairw%d:%d:%d", &pw, &a, &b) == 3) {
+      // The paired kernel with pw polling waves, whatever the grid.
+      const KernelEntry *k = dim == 3 ? pair_wide_kernel(a, b, tie, pw) : nullptr;
+      if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no wide paired kernel " + s);
+      if (plan_resident_shape(c, n, k, p, PlanKind::Pair) != FPSX_OK || p.grid < 2 ||
+          p.grid > (uint32_t)(64 * pw))
+        return fail(FPSX_ETOOBIG, "FPSX_PLAN: the grid does not fit " + s);
+      return FPSX_OK;
+    }
     if (std::sscanf(s.c_str(), "pair:%d:%d", &a, &b) == 2) {
       const KernelEntry *k = find_pair((int)dim, a, b, tie);
       if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no paired kernel for dim " +
                                            std::to_string(dim) + " shape " + s);
-      if (plan_resident_shape(c, n, k, p, PlanKind::Pair) != FPSX_OK || p.grid < 2 ||
-          p.grid > kPairMaxGrid)
-        return fail(FPSX_ETOOBIG, "FPSX_PLAN: the paired kernel needs 2..64 workgroups: " + s);
+      if (plan_resident_shape(c, n, k, p, PlanKind::Pair) == FPSX_OK && p.grid > kPairMaxGrid) {
+        // Beyond one polling wave's 64 workgroups: the form with 2 or 4.
+        const KernelEntry *kw = pair_wide_kernel(a, b, tie, p.grid <= 128 ? 2 : 4);
+        if (kw && plan_resident_shape(c, n, kw, p, PlanKind::Pair) == FPSX_OK &&
+            p.grid <= kPairWideMaxGrid)
+          return FPSX_OK;
+        return fail(FPSX_ETOOBIG, "FPSX_PLAN: the paired kernel needs 2..256 workgroups: " + s);
+      }
+      if (p.k != k || p.grid < 2)
+        return fail(FPSX_ETOOBIG, "FPSX_PLAN: the paired kernel needs 2..256 workgroups: " + s);
       return FPSX_OK;
     }
     if (std::sscanf(s.c_str(), "pruned:%d:%d", &a, &b) == 2) {
@@ -314,8 +331,16 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
             p.grid <= kPairMaxGrid)
           return FPSX_OK;
       }
+      // Beyond 64 workgroups (more than 1 M points): 1024 x 16 with two or
+      // four polling waves, up to 256 workgroups.
+      for (int pw : {2, 4}) {
+        const KernelEntry *k = pair_wide_kernel(1024, 16, tie, pw);
+        if (k && plan_resident_shape(c, n, k, p, PlanKind::Pair) == FPSX_OK &&
+            p.grid > kPairMaxGrid && p.grid <= (uint32_t)(64 * pw))
+          return FPSX_OK;
+      }
     }
-    // One pick per exchange (beyond 64 workgroups, or FPSX_NO_PAIR=1): from
+    // One pick per exchange (beyond 256 workgroups, or FPSX_NO_PAIR=1): from
     // 64 K points, where it beats the resident kernel.
     static const int kPr[][2] = {{1024, 16}, {256, 32}, {1024, 8}, {1024, 4}};
     for (auto &sh : kPr) {
@@ -516,6 +541,12 @@ static std::vector<Plan> candidates(Ctx &c, size_t n, size_t dim, int tie, size_
           plan_resident_shape(c, n, &t[i], q, PlanKind::Pair) == FPSX_OK && q.grid >= 2 &&
           q.grid <= kPairMaxGrid)
         out.push_back(q);
+    for (int pw : {2, 4}) {
+      const KernelEntry *k = pair_wide_kernel(1024, 16, tie, pw);
+      if (k && plan_resident_shape(c, n, k, q, PlanKind::Pair) == FPSX_OK && q.grid > kPairMaxGrid &&
+          q.grid <= (uint32_t)(64 * pw))
+        out.push_back(q);
+    }
     if (plan_stream_pruned(c, n, tie, 0, q) == FPSX_OK) out.push_back(q);
   }
   for (int B : {256, 1024}) {

@@ -72,10 +72,12 @@ void *fpsx_stream(void);
  * picks under `tie`: kind 0 = register-resident, 1 = HBM-streaming,
  * 2 = register-resident with spatial order and per-wave pruning,
  * 4 = HBM-streaming with spatial order and per-block pruning,
- * 5 = kind 2 settling up to two picks per cross-workgroup exchange (2..64
- * workgroups); threads per workgroup, points per thread (0 when streaming),
- * workgroups, points per workgroup.  FPSX_PLAN in the environment forces a
- * plan: "resident:B:PPT", "pruned:B:PPT", "pair:B:PPT", "stream:B[:G]",
+ * 5 = kind 2 settling up to two picks per cross-workgroup exchange (2..256
+ * workgroups; beyond 64, 1024 x 16 with 2 or 4 waves polling the exchange);
+ * threads per workgroup, points per thread (0 when streaming), workgroups,
+ * points per workgroup.  FPSX_PLAN in the environment forces a plan:
+ * "resident:B:PPT", "pruned:B:PPT", "pair:B:PPT", "pairwPW:B:PPT" (the paired
+ * kernel with PW = 2 or 4 polling waves at any grid it holds), "stream:B[:G]",
  * "generic:B[:G]", "spruned[:G]";
  * FPSX_NO_PRUNE=1 keeps the planner off the pruned kinds, FPSX_NO_PAIR=1 off
  * kind 5.

@@ -134,7 +134,7 @@ def test_pair_rejects_grids_outside_range(gpu, plan_env):
     plan_env("pair:1024:16")
     pc = np.random.default_rng(45).random((9000, 3)).astype(np.float32)
     rc, _ = gpu.fps(pc, 100, 0)
-    assert rc != 0 and "2..64" in gpu.last_error()
+    assert rc != 0 and "2..256" in gpu.last_error()  # one workgroup: nothing to pair across
 
 
 def test_pair_random_configurations(gpu, plan_env):
@@ -174,3 +174,45 @@ def test_pair_random_configurations(gpu, plan_env):
             rc, got = gpu.kdline(pc, 2000, h, 11)
             assert rc == 0, gpu.last_error()
             np.testing.assert_array_equal(got, want, err_msg=f"bucket case {case}")
+
+
+@pytest.mark.parametrize("pw", [2, 4])
+def test_wide_pair_small_grids(gpu, plan_env, pw):
+    # The several-polling-wave form forced on grids of 2..40 workgroups (the
+    # other polling waves see no workgroup): clustered, uniform, duplicate-
+    # heavy and special-value clouds, multi-start prefixes, both tie rules.
+    plan_env(f"pairw{pw}:1024:16")
+    rng = np.random.default_rng(60 + pw)
+    big = (rng.random((70000, 3)) * 4e19).astype(np.float32)
+    big[rng.integers(0, 70000, 3)] = np.nan
+    clouds = [("gauss", gauss_cloud(600_000, 5), [0]),
+              ("lattice_dups", np.repeat(rng.integers(0, 7, (12000, 3)), 4, axis=0).astype(np.float32), [5, 7]),
+              ("uniform", rng.random((33000, 3)).astype(np.float32), [32999]),
+              ("special", big, [1])]
+    for name, pc, st in clouds:
+        for m in (700, 1201):
+            log(f"pairw{pw} {name} n={len(pc)} m={m}")
+            np.testing.assert_array_equal(run_fps(gpu, pc, m, st), O.fps(pc, m, st),
+                                          err_msg=f"pairw{pw} {name} m={m}")
+    # lowest-position ties (the bucket entry's rule) through the KD-line order
+    pc = np.repeat(rng.integers(0, 5, (9000, 3)), 3, axis=0).astype(np.float32)
+    rc, got = gpu.kdline(pc, 900, 4, 4)
+    assert rc == 0, gpu.last_error()
+    np.testing.assert_array_equal(got, O.kdline_quickfps(pc, 900, 4, 4))
+
+
+@pytest.mark.parametrize("n,plan", [(1_200_000, None), (1_200_000, "pairw4:1024:16"),
+                                    (3_000_000, None)])
+def test_wide_pair_beyond_1m(gpu, plan_env, n, plan):
+    # 74 and 184 workgroups: the planner's 2- and 4-wave forms (and 4 waves
+    # forced at 74), oracle-compared on a prefix; the clouds have clusters
+    # and 10 % exact duplicates.
+    plan_env(plan)
+    pc = gauss_cloud(n, 9)
+    pc[::10] = pc[1::10][: len(pc[::10])]
+    m = 500
+    info = gpu.plan_info(n, 3, m=16384)
+    log(f"n={n} plan={plan} planner(16384)={info}")
+    if plan is None:
+        assert info["kind"] == 5 and info["grid"] > 64
+    np.testing.assert_array_equal(run_fps(gpu, pc, m, [0]), O.fps(pc, m, 0), err_msg=f"n={n} {plan}")
