@@ -85,6 +85,11 @@ ORACLE_API int oracle_fps_ex(const float *pts, size_t n, size_t d, size_t m,
 {
     if (m == 0)
         return 0;
+    if (!pts || !out || !starts || n == 0 || d == 0 || n_starts == 0)
+        return -1;
+    for (size_t k = 0; k < n_starts && k < m; ++k)
+        if (starts[k] >= n)
+            return -1; /* a start index outside the cloud */
     float *dmin = (float *)malloc(n * sizeof(float));
     if (!dmin)
         return -1;
@@ -147,6 +152,8 @@ ORACLE_API int oracle_npdu(const float *pts, size_t n, size_t d, size_t m, size_
 {
     if (m == 0)
         return 0;
+    if (!pts || !out || n == 0 || d == 0 || start >= n)
+        return -1;
     float *dmin = (float *)malloc(n * sizeof(float));
     if (!dmin)
         return -1;
@@ -223,6 +230,8 @@ ORACLE_API int oracle_fps_f64(const float *pts, size_t n, size_t d, size_t m, si
 {
     if (m == 0)
         return 0;
+    if (!pts || !out || n == 0 || d == 0 || start >= n)
+        return -1;
     double *dmin = (double *)malloc(n * sizeof(double));
     if (!dmin)
         return -1;
@@ -256,11 +265,17 @@ ORACLE_API int oracle_fps_f64(const float *pts, size_t n, size_t d, size_t m, si
 /*
  * Coverage radius of a pick set, in double: max over points of the Euclidean
  * distance to the nearest pick (the quantity farthest point sampling
- * greedily shrinks).  Returns -1.0 on allocation failure.
+ * greedily shrinks).  Returns -1.0 for a pick outside the cloud or on
+ * allocation failure.
  */
 ORACLE_API double oracle_coverage_f64(const float *pts, size_t n, size_t d, const uint64_t *picks,
                                       size_t k)
 {
+    if (!pts || n == 0 || d == 0 || (k && !picks))
+        return -1.0;
+    for (size_t s = 0; s < k; ++s)
+        if (picks[s] >= n)
+            return -1.0; /* a pick outside the cloud */
     double *dmin = (double *)malloc(n * sizeof(double));
     if (!dmin)
         return -1.0;
@@ -490,6 +505,8 @@ static int kd_build_tree(kd_build *kb, const float *pts, size_t n, size_t d, siz
 ORACLE_API long oracle_kdline_perm(const float *pts, size_t n, size_t d, size_t h,
                                    uint64_t *perm, uint64_t *leaf_lo, uint64_t *leaf_hi)
 {
+    if (!pts || !perm || n == 0 || d == 0)
+        return -1;
     kd_build kb;
     if (kd_build_tree(&kb, pts, n, d, h) != 0) {
         kd_free(&kb);
@@ -547,6 +564,8 @@ static inline float bucket_bound(const float *ref, const float *bmin, const floa
 ORACLE_API int oracle_kdline_quickfps(const float *pts, size_t n, size_t d, size_t m,
                                       size_t start, size_t h, uint64_t *out)
 {
+    if (!pts || !out || n == 0 || d == 0 || start >= n)
+        return -1;
     kd_build kb;
     if (kd_build_tree(&kb, pts, n, d, h) != 0) {
         kd_free(&kb);
@@ -666,6 +685,8 @@ done:
 ORACLE_API int oracle_kdline_exhaustive(const float *pts, size_t n, size_t d, size_t m,
                                         size_t start, size_t h, uint64_t *out)
 {
+    if (!pts || !out || n == 0 || d == 0 || start >= n)
+        return -1;
     kd_build kb;
     if (kd_build_tree(&kb, pts, n, d, h) != 0) {
         kd_free(&kb);

@@ -102,7 +102,7 @@ def fps_ex(pc: np.ndarray, n_samples: int, start_idx=0, lowest: bool = False,
         st.ctypes.data_as(_u64p), st.size, int(bool(lowest)), ctypes.c_float(init),
         out.ctypes.data_as(_u64p), dist.ctypes.data_as(_f32p) if with_dist else None)
     if rc != 0:
-        raise MemoryError("oracle_fps_ex failed")
+        raise RuntimeError("oracle_fps_ex: bad arguments or out of memory")
     return (out, dist) if with_dist else out
 
 
@@ -123,7 +123,7 @@ def kdline_perm(pc: np.ndarray, h: int):
                                   perm.ctypes.data_as(_u64p), lo.ctypes.data_as(_u64p),
                                   hi.ctypes.data_as(_u64p))
     if nl < 0:
-        raise MemoryError("oracle_kdline_perm failed")
+        raise RuntimeError("oracle_kdline_perm: bad arguments or out of memory")
     return perm, lo[:nl], hi[:nl]
 
 
@@ -133,7 +133,7 @@ def _kdline(fn, pc, n_samples, h, start_idx):
     rc = fn(pts.ctypes.data_as(_f32p), pts.shape[0], pts.shape[1], n_samples,
             int(start_idx), int(h), out.ctypes.data_as(_u64p))
     if rc != 0:
-        raise MemoryError("oracle kdline failed")
+        raise RuntimeError("oracle kdline: bad arguments or out of memory")
     return out
 
 
@@ -154,7 +154,7 @@ def npdu(pc: np.ndarray, n_samples: int, w: int, start_idx: int = 0) -> np.ndarr
     rc = lib().oracle_npdu(pts.ctypes.data_as(_f32p), pts.shape[0], pts.shape[1], n_samples,
                            int(w), int(start_idx), out.ctypes.data_as(_u64p))
     if rc != 0:
-        raise MemoryError("oracle_npdu failed")
+        raise RuntimeError("oracle_npdu: bad arguments or out of memory")
     return out
 
 
@@ -168,7 +168,7 @@ def fps_f64(pc: np.ndarray, n_samples: int, start_idx: int = 0):
                               int(start_idx), out.ctypes.data_as(_u64p),
                               dist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     if rc != 0:
-        raise MemoryError("oracle_fps_f64 failed")
+        raise RuntimeError("oracle_fps_f64: bad arguments or out of memory")
     return out, dist
 
 
@@ -179,7 +179,7 @@ def coverage_radius(pc: np.ndarray, picks) -> float:
     r = lib().oracle_coverage_f64(pts.ctypes.data_as(_f32p), pts.shape[0], pts.shape[1],
                                   p.ctypes.data_as(_u64p), p.size)
     if r < 0:
-        raise MemoryError("oracle_coverage_f64 failed")
+        raise RuntimeError("oracle_coverage_f64: bad arguments or out of memory")
     return float(r)
 
 
