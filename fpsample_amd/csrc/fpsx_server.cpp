@@ -17,6 +17,13 @@
 //               for the host between chunks;
 //   retire   -- once a launch completes, the requests it finished copy their
 //               picks back, free their range and wake their waiters.
+// Request control: pending requests are admitted by priority (FIFO among
+// equals); each launch takes the admitted requests in priority order while
+// their workgroups fit the grid, so a lower-priority request is paused
+// between chunks (its state stays in its range) when higher ones need the
+// room.  A cancelled request, or one past its deadline, leaves the queue at
+// once or, when running, stops at its next chunk boundary: its range is freed
+// once no queued launch still writes it.
 // Each request's picks equal fps_sampling on its cloud alone: a job's result
 // never depends on what else shares the launch (exact min and argmax, ties to
 // the highest index), and a chunk boundary only saves and restores state.
@@ -110,6 +117,12 @@ struct Request {
   hipEvent_t uploaded = nullptr;
   bool ready = false, done = false;
   bool finishing = false;  // its last chunk is queued: in no further launch
+  uint32_t inflight = 0;   // queued launches that include it
+  int priority = 0;
+  bool has_deadline = false;
+  Clock::time_point deadline;
+  bool stop = false;        // cancelled or past its deadline: no further chunk
+  int stop_code = FPSX_OK;  // FPSX_ECANCELED or FPSX_EDEADLINE
   int error = FPSX_OK;
   std::vector<uint64_t> picks;
   Clock::time_point t_submit, t_admit, t_done;
@@ -158,6 +171,7 @@ struct fpsx_server {
   Clock::time_point t0 = Clock::now();
   // counters
   uint64_t launches = 0, completed = 0, samples_done = 0;
+  uint64_t n_cancelled = 0, n_expired = 0, n_paused = 0;
   double busy_s = 0;
   int fatal = FPSX_OK;
   std::string fatal_msg;
@@ -175,6 +189,17 @@ struct fpsx_server {
     if (pruned) b += align_up((size_t)r.n * r.dim * sizeof(float)) + align_up((size_t)r.n * sizeof(uint32_t));
     return b;
   }
+  // Ends a request that will not finish (caller holds mu; wakes no one).  Its
+  // host copy is no longer read by any upload.
+  void end_request(Request &r, int code, Clock::time_point now) {
+    r.soa.clear();
+    r.soa.shrink_to_fit();
+    r.done = true;
+    r.error = code;
+    r.t_done = now;
+    if (code == FPSX_ECANCELED) ++n_cancelled;
+    if (code == FPSX_EDEADLINE) ++n_expired;
+  }
   void loop();
   int step();  // one admit / launch / retire round; caller does not hold mu
   int finish(Launch &L);  // wait for a queued launch, check it, retire what it finished
@@ -182,15 +207,34 @@ struct fpsx_server {
 
 int fpsx_server::step() {
   std::vector<std::shared_ptr<Request>> admitted;
+  bool ended = false;  // a request ended without finishing: wake waiters
   {
     std::unique_lock<std::mutex> lk(mu);
-    // Launches run one after another, so only the requests still to be
-    // launched count against the co-resident grid.
-    uint32_t members = 0;
+    const auto now = Clock::now();
+    // Deadlines: a queued request leaves the queue, a running one stops at
+    // this chunk boundary unless its last chunk is already queued.
+    for (auto it = pending.begin(); it != pending.end();) {
+      if ((*it)->has_deadline && now >= (*it)->deadline) {
+        end_request(**it, FPSX_EDEADLINE, now);
+        it = pending.erase(it);
+        ended = true;
+      } else {
+        ++it;
+      }
+    }
     for (auto &r : active)
-      if (!r->finishing) members += r->team;
+      if (!r->stop && !r->finishing && r->has_deadline && now >= r->deadline) {
+        r->stop = true;
+        r->stop_code = FPSX_EDEADLINE;
+      }
     while (!pending.empty()) {
       auto &r = pending.front();
+      // Launches run one after another, so only the requests still to be
+      // launched count against the co-resident grid, and of those only the
+      // ones this request cannot preempt (its priority or higher).
+      uint32_t members = 0;
+      for (auto &a : active)
+        if (!a->finishing && !a->stop && a->priority >= r->priority) members += a->team;
       if (members + r->team > grid_cap) break;
       size_t off;
       if (!arena->alloc(r->bytes, &off)) break;  // wait for memory to free up
@@ -200,8 +244,7 @@ int fpsx_server::step() {
       r->out_off = r->state_off + align_up(((size_t)r->n + 4) * sizeof(float));
       r->sorted_off = r->out_off + align_up((size_t)r->m * sizeof(uint64_t));
       r->perm_off = r->sorted_off + align_up((size_t)r->n * r->dim * sizeof(float));
-      r->t_admit = Clock::now();
-      members += r->team;
+      r->t_admit = now;
       admitted.push_back(r);
       active.push_back(r);
       pending.pop_front();
@@ -222,15 +265,38 @@ int fpsx_server::step() {
     if (e == hipSuccess) e = hipEventRecord(r->uploaded, copy);
     if (e != hipSuccess) return fail(FPSX_EHIP, std::string("upload: ") + hipGetErrorString(e));
   }
-  // Jobs of the next launch: every uploaded request not yet in its last
-  // chunk, from where the queued launches leave it.
+  // Jobs of the next launch: the uploaded requests not yet in their last
+  // chunk, highest priority first, while their teams fit the grid (the rest
+  // are paused for this launch), each from where the queued launches leave it.
   std::vector<std::shared_ptr<Request>> in_launch;
   std::vector<JobSpec> specs;
   bool waiting_upload = false;
   {
     std::lock_guard<std::mutex> lk(mu);
-    for (auto &r : active) {
+    std::vector<std::shared_ptr<Request>> order(active);
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::shared_ptr<Request> &a, const std::shared_ptr<Request> &b) {
+                       return a->priority > b->priority;
+                     });
+    uint32_t members = 0;
+    for (auto &r : order) {
       if (r->finishing) continue;
+      if (r->stop) {
+        // Retire once no queued launch writes its range and its upload
+        // (points and spatial order on the copy stream) is complete.
+        if (r->inflight) continue;
+        if (!r->ready && hipEventQuery(r->uploaded) != hipSuccess) {
+          waiting_upload = true;
+          continue;
+        }
+        arena->release(r->off, r->bytes);
+        if (r->uploaded) hipEventDestroy(r->uploaded);
+        r->uploaded = nullptr;
+        active.erase(std::find(active.begin(), active.end(), r));
+        end_request(*r, r->stop_code, Clock::now());
+        ended = true;
+        continue;
+      }
       if (!r->ready) {
         if (hipEventQuery(r->uploaded) != hipSuccess) {
           waiting_upload = true;
@@ -252,10 +318,20 @@ int fpsx_server::step() {
       s.s_end = std::min<uint32_t>(r->m, (r->next == 0 ? 1u : r->next) + chunk);
       s.sorted_off = r->sorted_off / sizeof(float);
       s.perm_off = r->perm_off / sizeof(uint32_t);
+      if (members + r->team > grid_cap) {
+        ++n_paused;  // preempted by higher-priority requests for this launch
+        continue;
+      }
+      members += r->team;
+      // Committed to this launch (a failure to launch stops the server).
+      r->next = s.s_end;
+      if (s.s_end >= r->m) r->finishing = true;
+      ++r->inflight;
       specs.push_back(s);
       in_launch.push_back(r);
     }
   }
+  if (ended) cv_done.notify_all();
   Launch &head = slot[cur];
   if (specs.empty()) {
     if (head.busy) return finish(head);
@@ -291,11 +367,7 @@ int fpsx_server::step() {
   if (e != hipSuccess) return fail(FPSX_EHIP, std::string("serve launch: ") + hipGetErrorString(e));
   L.reqs = in_launch;
   L.s_end.resize(specs.size());
-  for (size_t i = 0; i < specs.size(); ++i) {
-    L.s_end[i] = specs[i].s_end;
-    in_launch[i]->next = specs[i].s_end;
-    if (specs[i].s_end >= in_launch[i]->m) in_launch[i]->finishing = true;
-  }
+  for (size_t i = 0; i < specs.size(); ++i) L.s_end[i] = specs[i].s_end;
   L.busy = true;
   // With a launch queued behind it, wait for the older one and retire.
   if (&L != &head) return finish(head);
@@ -316,6 +388,10 @@ int fpsx_server::finish(Launch &L) {
   ++launches;
   // Retire the requests this launch finished.
   std::vector<std::shared_ptr<Request>> finished;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &r : L.reqs) --r->inflight;
+  }
   for (size_t i = 0; i < L.reqs.size(); ++i)
     if (L.s_end[i] >= L.reqs[i]->m) finished.push_back(L.reqs[i]);
   L.reqs.clear();
@@ -445,9 +521,11 @@ FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passe
   return FPSX_OK;
 }
 
-FPSX_EXPORT int fpsx_server_submit(fpsx_server *s, const float *points, size_t n_points, size_t dim,
-                                   size_t n_samples, uint64_t start_idx, uint64_t *request_id) {
+FPSX_EXPORT int fpsx_server_submit_ex(fpsx_server *s, const float *points, size_t n_points,
+                                      size_t dim, size_t n_samples, uint64_t start_idx,
+                                      int priority, double deadline_s, uint64_t *request_id) {
   if (!s || !points || !request_id) return fail(FPSX_EINVAL, "null argument");
+  if (std::isnan(deadline_s)) return fail(FPSX_EINVAL, "deadline is NaN");
   if (dim != 3) return fail(FPSX_EBADDIM, "the serving loop takes 3-D clouds");
   if (n_points == 0 || n_points >= (size_t(1) << 31)) return fail(FPSX_ETOOBIG, "bad n_points");
   if (n_samples == 0 || n_samples > n_points) return fail(FPSX_EINVAL, "n_samples must be in [1, n]");
@@ -467,33 +545,107 @@ FPSX_EXPORT int fpsx_server_submit(fpsx_server *s, const float *points, size_t n
   r->soa.resize(n_points * dim);
   for (size_t i = 0; i < n_points; ++i)
     for (size_t d = 0; d < dim; ++d) r->soa[d * n_points + i] = points[i * dim + d];
+  r->priority = priority;
   r->t_submit = Clock::now();
+  if (deadline_s > 0) {
+    r->has_deadline = true;
+    // (capped at about 30 years so the time point cannot overflow)
+    r->deadline = r->t_submit + std::chrono::duration_cast<Clock::duration>(
+                                    std::chrono::duration<double>(std::min(deadline_s, 1e9)));
+  }
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->fatal) return fail(s->fatal, "server stopped: " + s->fatal_msg);
     r->id = s->next_id++;
     s->all[r->id] = r;
-    s->pending.push_back(r);
+    // Behind every pending request of its priority or higher.
+    auto pos = std::find_if(s->pending.begin(), s->pending.end(),
+                            [&](const std::shared_ptr<Request> &q) { return q->priority < priority; });
+    s->pending.insert(pos, r);
   }
   s->cv_work.notify_one();
   *request_id = r->id;
   return FPSX_OK;
 }
 
-FPSX_EXPORT int fpsx_server_wait(fpsx_server *s, uint64_t request_id, uint64_t *out,
-                                 double *latency_s) {
+FPSX_EXPORT int fpsx_server_submit(fpsx_server *s, const float *points, size_t n_points, size_t dim,
+                                   size_t n_samples, uint64_t start_idx, uint64_t *request_id) {
+  return fpsx_server_submit_ex(s, points, n_points, dim, n_samples, start_idx, 0, 0.0, request_id);
+}
+
+FPSX_EXPORT int fpsx_server_wait_for(fpsx_server *s, uint64_t request_id, double timeout_s,
+                                     uint64_t *out, double *latency_s) {
   if (!s) return fail(FPSX_EINVAL, "null server");
+  if (std::isnan(timeout_s)) return fail(FPSX_EINVAL, "timeout is NaN");
   std::shared_ptr<Request> r;
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->all.find(request_id);
   if (it == s->all.end()) return fail(FPSX_EINVAL, "unknown request id");
   r = it->second;
-  s->cv_done.wait(lk, [&] { return r->done; });
+  auto done = [&] { return r->done; };
+  if (timeout_s < 0) {
+    s->cv_done.wait(lk, done);
+  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(std::min(timeout_s, 1e9)),
+                                  done)) {
+    return fail(FPSX_EPENDING, "request " + std::to_string(request_id) + " still running");
+  }
   s->all.erase(request_id);
+  if (r->error == FPSX_ECANCELED) return fail(r->error, "request was cancelled");
+  if (r->error == FPSX_EDEADLINE) return fail(r->error, "request's deadline passed");
   if (r->error) return fail(r->error, "request failed: " + s->fatal_msg);
   if (out) std::memcpy(out, r->picks.data(), r->m * sizeof(uint64_t));
   if (latency_s) *latency_s = secs_since(r->t_submit, r->t_done);
   return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_server_wait(fpsx_server *s, uint64_t request_id, uint64_t *out,
+                                 double *latency_s) {
+  return fpsx_server_wait_for(s, request_id, -1.0, out, latency_s);
+}
+
+FPSX_EXPORT int fpsx_server_cancel(fpsx_server *s, uint64_t request_id, int *cancelled) {
+  if (cancelled) *cancelled = 0;
+  if (!s) return fail(FPSX_EINVAL, "null server");
+  bool wake_waiters = false, wake_loop = false;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(request_id);
+    if (it == s->all.end()) return fail(FPSX_EINVAL, "unknown request id");
+    Request &r = *it->second;
+    if (r.done) {
+      // finished (or already ended): nothing to stop
+      if (cancelled) *cancelled = (r.error != FPSX_OK);
+      return FPSX_OK;
+    }
+    auto q = std::find(s->pending.begin(), s->pending.end(), it->second);
+    if (q != s->pending.end()) {
+      s->pending.erase(q);
+      s->end_request(r, FPSX_ECANCELED, Clock::now());
+      wake_waiters = true;
+    } else if (r.finishing) {
+      return FPSX_OK;  // its last chunk is queued: it finishes
+    } else {
+      if (!r.stop) {
+        r.stop = true;
+        r.stop_code = FPSX_ECANCELED;
+      }
+      wake_loop = true;
+    }
+    if (cancelled) *cancelled = 1;
+  }
+  if (wake_waiters) s->cv_done.notify_all();
+  if (wake_loop) s->cv_work.notify_one();
+  return FPSX_OK;
+}
+
+FPSX_EXPORT int fpsx_server_counters(fpsx_server *s, uint64_t *cancelled, uint64_t *expired,
+                                     uint64_t *paused) {
+  if (!s) return fail(FPSX_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (cancelled) *cancelled = s->n_cancelled;
+  if (expired) *expired = s->n_expired;
+  if (paused) *paused = s->n_paused;
+  return s->fatal;
 }
 
 FPSX_EXPORT int fpsx_server_stats(fpsx_server *s, uint64_t *launches, uint64_t *completed,

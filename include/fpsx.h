@@ -29,6 +29,9 @@ extern "C" {
 #define FPSX_ENOMEM 6     /* device or host allocation failed */
 #define FPSX_ETIMEOUT 7   /* a cross-workgroup exchange did not complete (kernel gave up) */
 #define FPSX_EBUDGET 8    /* serving: request refused by the HBM budget */
+#define FPSX_ECANCELED 9  /* serving: the request was cancelled before it finished */
+#define FPSX_EPENDING 10  /* serving: fpsx_server_wait_for timed out; the request still runs */
+#define FPSX_EDEADLINE 11 /* serving: the request's deadline passed before it finished */
 
 /* Point layouts accepted by the host entry points. */
 #define FPSX_LAYOUT_ROWS 0 /* row-major n x dim (C order): point i at p[i*dim + j] */
@@ -317,6 +320,43 @@ int fpsx_server_wait(fpsx_server *s, uint64_t request_id, uint64_t *out, double 
 int fpsx_server_stats(fpsx_server *s, uint64_t *launches, uint64_t *completed, uint64_t *samples,
                       double *busy_s, size_t *hbm_in_use, size_t *pending, size_t *active);
 void fpsx_server_destroy(fpsx_server *s);
+
+/*
+ * Request control (no counterpart in the reference, which has no serving loop).
+ *
+ * fpsx_server_submit_ex: fpsx_server_submit with a priority and a deadline.
+ *   Pending requests are admitted highest priority first (FIFO among equals;
+ *   fpsx_server_submit is priority 0).  Priorities also preempt: each launch
+ *   takes the admitted requests in priority order while their workgroups fit
+ *   the co-resident grid, and a lower-priority request that no longer fits is
+ *   paused between two chunks -- its min-distances stay in its HBM range --
+ *   and resumes, bit-identically, once the grid has room.  A request is
+ *   admitted while the requests of its priority or higher leave room in the
+ *   grid.  Paused requests keep their HBM (no eviction to host memory).
+ *   deadline_s > 0: seconds after submission by which the request must have
+ *   finished; past it, a request not yet in its last chunk is stopped at the
+ *   next chunk boundary (or dropped from the queue) and its wait returns
+ *   FPSX_EDEADLINE.  deadline_s <= 0: no deadline.
+ * fpsx_server_cancel: stops a request at its next chunk boundary (or drops it
+ *   from the queue) and frees its HBM; its wait returns FPSX_ECANCELED
+ *   (FPSX_EDEADLINE if its deadline had already stopped it).  *cancelled
+ *   (optional) is 1 if the request ends without picks, 0 if it had already
+ *   finished or its last chunk was already queued (its wait returns the
+ *   picks).  FPSX_EINVAL for an id that is unknown or already waited for.
+ * fpsx_server_wait_for: fpsx_server_wait with a time limit in seconds (< 0:
+ *   none, 0: poll).  FPSX_EPENDING when the limit passes first; the request
+ *   stays registered and can be waited for or cancelled again.
+ * fpsx_server_counters: requests cancelled, requests stopped by their
+ *   deadline, and (request, launch) pairs in which a request was paused.
+ */
+int fpsx_server_submit_ex(fpsx_server *s, const float *points, size_t n_points, size_t dim,
+                          size_t n_samples, uint64_t start_idx, int priority, double deadline_s,
+                          uint64_t *request_id);
+int fpsx_server_cancel(fpsx_server *s, uint64_t request_id, int *cancelled);
+int fpsx_server_wait_for(fpsx_server *s, uint64_t request_id, double timeout_s, uint64_t *out,
+                         double *latency_s);
+int fpsx_server_counters(fpsx_server *s, uint64_t *cancelled, uint64_t *expired,
+                         uint64_t *paused);
 
 /*
  * Bucket FPS entry with the exact signature and return codes of the
