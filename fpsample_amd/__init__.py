@@ -62,16 +62,17 @@ def get_start_idx(n_pts: int, start_idx: Optional[Union[int, List[int]]]) -> Uni
 def fps_sampling(
     pc: np.ndarray, n_samples: int, start_idx: Optional[Union[int, List[int]]] = None
 ) -> np.ndarray:
-    """
-    Vanilla FPS sampling (src/fpsample/__init__.py:35-65), run on the GPU.
+    """Farthest point sampling on the GPU, with the picks of src/fpsample/__init__.py:35-65.
 
-    Args:
-        pc (np.ndarray): The input point cloud of shape (n_pts, D).
-        n_samples (int): Number of samples.
-        start_idx (int or list[int], default=None): The starting index of sampling. If set to None, it will be randomly picked.
-            If multiple start indices are given, they are used as a start set for the FPS and will all be present in the final samples.
-    Returns:
-        np.ndarray: The selected indices of shape (n_samples,), dtype uint64.
+    Each pick after the first is the point whose squared distance to its
+    nearest earlier pick is largest (the highest index among equal maxima).
+
+    pc: (n_pts, D) array, any float dtype (converted to float32).
+    n_samples: how many indices to return, 1 <= n_samples <= n_pts.
+    start_idx: the first pick; None draws one from numpy's global RNG; a list
+        forces that prefix of picks (at most n_samples, each < n_pts), after
+        which sampling continues from all of them.
+    Returns a uint64 array of n_samples point indices, in pick order.
     """
     assert n_samples >= 1, "n_samples should be >= 1"
     assert pc.ndim == 2
@@ -105,13 +106,12 @@ def fps_npdu_sampling(
     index order (dimensional locality); the next pick is still the largest
     min-distance over all points, lowest index on ties.
 
-    Args:
-        pc (np.ndarray): The input point cloud of shape (n_pts, D).
-        n_samples (int): Number of samples.
-        w (int, default=None): Window of the local update; None means ``int(n_pts / n_samples * 16)``.
-        start_idx (int, default=None): The starting index of sampling; random if None.
-    Returns:
-        np.ndarray: The selected indices of shape (n_samples,), dtype uint64.
+    pc: (n_pts, D) array (converted to float32, row-major).
+    n_samples: how many indices to return, 1 <= n_samples <= n_pts.
+    w: update window; None means ``int(n_pts / n_samples * 16)``, and values of
+        n_pts - 1 or more are lowered to n_pts - 1 with a warning.
+    start_idx: the first pick; None draws one from numpy's global RNG.
+    Returns a uint64 array of n_samples point indices, in pick order.
     """
     assert n_samples >= 1, "n_samples should be >= 1"
     assert pc.ndim == 2
@@ -143,14 +143,12 @@ def bucket_fps_kdline_sampling(
     GPU as exhaustive FPS over the tree order, which selects the same points as
     the reference's pruned bucket search.
 
-    Args:
-        pc (np.ndarray): The input point cloud of shape (n_pts, D), 1 <= D <= 8.
-        n_samples (int): Number of samples.
-        h (int): Height of the KD tree (2**h buckets).
-        start_idx (int, default=None): Position of the first pick in the tree order; random if None.
-
-    Returns:
-        np.ndarray: The selected indices of shape (n_samples,), dtype uint64.
+    pc: (n_pts, D) array with 1 <= D <= 8 (converted to float32, row-major).
+    n_samples: how many indices to return, 1 <= n_samples <= n_pts.
+    h: tree height, 1 <= h and 2**h <= n_pts (2**h buckets).
+    start_idx: position of the first pick in the tree order; None draws one
+        from numpy's global RNG.
+    Returns a uint64 array of n_samples point indices, in pick order.
     """
     assert n_samples >= 1, "n_samples should be >= 1"
     assert pc.ndim == 2
@@ -179,13 +177,11 @@ def fps_sampling_batch(
     Each cloud's picks equal ``fps_sampling(pc, n, start_idx=s)`` for that cloud
     alone.  Clouds may differ in size but must share the number of columns.
 
-    Args:
-        pcs (list[np.ndarray]): point clouds of shape (n_i, D).
-        n_samples (int or list[int]): samples per cloud (one value for all, or one each).
-        start_idx (None, int or list[int]): first pick per cloud; None draws one
-            np.random.randint(0, n_i) per cloud, in order.
-    Returns:
-        list[np.ndarray]: uint64 picks per cloud.
+    pcs: list of (n_i, D) arrays.
+    n_samples: picks per cloud, one value for all or one per cloud.
+    start_idx: first pick per cloud (None draws np.random.randint(0, n_i) for
+        each cloud in order; an int applies to every cloud).
+    Returns a list of uint64 pick arrays, one per cloud.
     """
     assert isinstance(pcs, (list, tuple)) and len(pcs) >= 1, "pcs should be a non-empty list"
     k = len(pcs)
