@@ -21,9 +21,13 @@
 // equals); each launch takes the admitted requests in priority order while
 // their workgroups fit the grid, so a lower-priority request is paused
 // between chunks (its state stays in its range) when higher ones need the
-// room.  A cancelled request, or one past its deadline, leaves the queue at
-// once or, when running, stops at its next chunk boundary: its range is freed
-// once no queued launch still writes it.
+// room.  When a request's HBM is missing, lower-priority requests are evicted
+// at their next chunk boundary: the whole range (points, spatial order,
+// min-distances, picks so far) is copied to host memory and freed, and the
+// request waits in the queue to be copied back and resumed where it stopped.
+// A cancelled request, or one past its deadline, leaves the queue at once or,
+// when running, stops at its next chunk boundary: its range is freed once no
+// queued launch still writes it.
 // Each request's picks equal fps_sampling on its cloud alone: a job's result
 // never depends on what else shares the launch (exact min and argmax, ties to
 // the highest index), and a chunk boundary only saves and restores state.
@@ -95,6 +99,7 @@ class Arena {
     }
   }
   size_t used_bytes() const { return used_ * kAlign; }
+  size_t free_bytes() const { return (units_ - used_) * kAlign; }
   static constexpr size_t kAlign = 256;
 
  private:
@@ -123,6 +128,8 @@ struct Request {
   Clock::time_point deadline;
   bool stop = false;        // cancelled or past its deadline: no further chunk
   int stop_code = FPSX_OK;  // FPSX_ECANCELED or FPSX_EDEADLINE
+  bool evict = false;       // leaves the device at its next chunk boundary
+  std::vector<char> saved;  // evicted: the device range, restored on re-admission
   int error = FPSX_OK;
   std::vector<uint64_t> picks;
   Clock::time_point t_submit, t_admit, t_done;
@@ -171,7 +178,7 @@ struct fpsx_server {
   Clock::time_point t0 = Clock::now();
   // counters
   uint64_t launches = 0, completed = 0, samples_done = 0;
-  uint64_t n_cancelled = 0, n_expired = 0, n_paused = 0;
+  uint64_t n_cancelled = 0, n_expired = 0, n_paused = 0, n_evicted = 0;
   double busy_s = 0;
   int fatal = FPSX_OK;
   std::string fatal_msg;
@@ -194,11 +201,42 @@ struct fpsx_server {
   void end_request(Request &r, int code, Clock::time_point now) {
     r.soa.clear();
     r.soa.shrink_to_fit();
+    r.saved.clear();
+    r.saved.shrink_to_fit();
     r.done = true;
     r.error = code;
     r.t_done = now;
     if (code == FPSX_ECANCELED) ++n_cancelled;
     if (code == FPSX_EDEADLINE) ++n_expired;
+  }
+  // r's HBM is missing: mark lower-priority requests for eviction (lowest
+  // priority first, then the most recently admitted) until the free bytes
+  // plus theirs would hold it.  Caller holds mu.
+  void choose_victims(const Request &r) {
+    size_t coming = 0;
+    for (auto &a : active)
+      if (a->evict) coming += a->bytes;
+    if (coming) return;  // evictions under way: retry once they are done
+    std::vector<std::shared_ptr<Request>> cand;
+    for (auto &a : active)
+      if (a->priority < r.priority && !a->finishing && !a->stop && a->ready) cand.push_back(a);
+    std::sort(cand.begin(), cand.end(),
+              [](const std::shared_ptr<Request> &x, const std::shared_ptr<Request> &y) {
+                if (x->priority != y->priority) return x->priority < y->priority;
+                return x->t_admit > y->t_admit;
+              });
+    // (at least one victim: with enough free bytes, the range is fragmented)
+    for (auto &a : cand) {
+      a->evict = true;
+      coming += a->bytes;
+      if (arena->free_bytes() + coming >= r.bytes) break;
+    }
+  }
+  // Insert r into the queue behind the requests of its priority or higher.
+  void enqueue(const std::shared_ptr<Request> &r) {
+    auto pos = std::find_if(pending.begin(), pending.end(),
+                            [&](const std::shared_ptr<Request> &q) { return q->priority < r->priority; });
+    pending.insert(pos, r);
   }
   void loop();
   int step();  // one admit / launch / retire round; caller does not hold mu
@@ -237,7 +275,10 @@ int fpsx_server::step() {
         if (!a->finishing && !a->stop && a->priority >= r->priority) members += a->team;
       if (members + r->team > grid_cap) break;
       size_t off;
-      if (!arena->alloc(r->bytes, &off)) break;  // wait for memory to free up
+      if (!arena->alloc(r->bytes, &off)) {
+        choose_victims(*r);
+        break;  // wait for memory to free up
+      }
       r->off = off;
       r->pts_off = off;
       r->state_off = off + align_up((size_t)r->n * r->dim * sizeof(float));
@@ -250,9 +291,18 @@ int fpsx_server::step() {
       pending.pop_front();
     }
   }
-  // Upload newly admitted clouds on the copy stream (overlaps the kernel).
+  // Upload newly admitted clouds on the copy stream (overlaps the kernel); an
+  // evicted request gets its saved range back as it was.
   for (auto &r : admitted) {
     char *base = (char *)arena_base;
+    if (!r->saved.empty()) {
+      hipError_t e = hipMemcpyAsync(base + r->off, r->saved.data(), r->bytes,
+                                    hipMemcpyHostToDevice, copy);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(r->uploaded, copy);
+      if (e != hipSuccess) return fail(FPSX_EHIP, std::string("restore: ") + hipGetErrorString(e));
+      continue;
+    }
     hipError_t e = hipMemcpyAsync(base + r->pts_off, r->soa.data(), r->soa.size() * sizeof(float),
                                   hipMemcpyHostToDevice, copy);
     if (e == hipSuccess && pruned) {
@@ -268,7 +318,7 @@ int fpsx_server::step() {
   // Jobs of the next launch: the uploaded requests not yet in their last
   // chunk, highest priority first, while their teams fit the grid (the rest
   // are paused for this launch), each from where the queued launches leave it.
-  std::vector<std::shared_ptr<Request>> in_launch;
+  std::vector<std::shared_ptr<Request>> in_launch, leaving;
   std::vector<JobSpec> specs;
   bool waiting_upload = false;
   {
@@ -297,6 +347,10 @@ int fpsx_server::step() {
         ended = true;
         continue;
       }
+      if (r->evict) {
+        if (!r->inflight) leaving.push_back(r);  // no queued launch writes its range
+        continue;
+      }
       if (!r->ready) {
         if (hipEventQuery(r->uploaded) != hipSuccess) {
           waiting_upload = true;
@@ -305,6 +359,8 @@ int fpsx_server::step() {
         r->ready = true;
         r->soa.clear();
         r->soa.shrink_to_fit();
+        r->saved.clear();
+        r->saved.shrink_to_fit();
       }
       JobSpec s;
       s.pt_off = r->pts_off / sizeof(float);
@@ -329,6 +385,32 @@ int fpsx_server::step() {
       ++r->inflight;
       specs.push_back(s);
       in_launch.push_back(r);
+    }
+  }
+  // Evictions: copy each range to host memory (nothing in flight writes it),
+  // free it and queue the request again at its priority.
+  for (auto &r : leaving) {
+    r->saved.resize(r->bytes);
+    hipError_t e = hipMemcpy(r->saved.data(), (char *)arena_base + r->off, r->bytes,
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(FPSX_EHIP, std::string("evict: ") + hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(mu);
+    arena->release(r->off, r->bytes);
+    if (r->uploaded) hipEventDestroy(r->uploaded);
+    r->uploaded = nullptr;
+    r->ready = false;
+    r->evict = false;
+    active.erase(std::find(active.begin(), active.end(), r));
+    ++n_evicted;
+    if (r->stop) {  // cancelled or expired meanwhile
+      end_request(*r, r->stop_code, Clock::now());
+      ended = true;
+    } else {
+      // ahead of the requests of its priority that never ran
+      auto pos = std::find_if(pending.begin(), pending.end(), [&](const std::shared_ptr<Request> &q) {
+        return q->priority < r->priority || (q->priority == r->priority && q->saved.empty());
+      });
+      pending.insert(pos, r);
     }
   }
   if (ended) cv_done.notify_all();
@@ -558,10 +640,7 @@ FPSX_EXPORT int fpsx_server_submit_ex(fpsx_server *s, const float *points, size_
     if (s->fatal) return fail(s->fatal, "server stopped: " + s->fatal_msg);
     r->id = s->next_id++;
     s->all[r->id] = r;
-    // Behind every pending request of its priority or higher.
-    auto pos = std::find_if(s->pending.begin(), s->pending.end(),
-                            [&](const std::shared_ptr<Request> &q) { return q->priority < priority; });
-    s->pending.insert(pos, r);
+    s->enqueue(r);  // behind every pending request of its priority or higher
   }
   s->cv_work.notify_one();
   *request_id = r->id;
@@ -639,12 +718,13 @@ FPSX_EXPORT int fpsx_server_cancel(fpsx_server *s, uint64_t request_id, int *can
 }
 
 FPSX_EXPORT int fpsx_server_counters(fpsx_server *s, uint64_t *cancelled, uint64_t *expired,
-                                     uint64_t *paused) {
+                                     uint64_t *paused, uint64_t *evicted) {
   if (!s) return fail(FPSX_EINVAL, "null server");
   std::lock_guard<std::mutex> lk(s->mu);
   if (cancelled) *cancelled = s->n_cancelled;
   if (expired) *expired = s->n_expired;
   if (paused) *paused = s->n_paused;
+  if (evicted) *evicted = s->n_evicted;
   return s->fatal;
 }
 

@@ -33,7 +33,7 @@ def _lib():
         L.fpsx_server_wait_for.argtypes = [vp, u64, ctypes.c_double, u64p,
                                            ctypes.POINTER(ctypes.c_double)]
         L.fpsx_server_cancel.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int)]
-        L.fpsx_server_counters.argtypes = [vp, u64p, u64p, u64p]
+        L.fpsx_server_counters.argtypes = [vp, u64p, u64p, u64p, u64p]
         for f in ("fpsx_server_create", "fpsx_server_submit", "fpsx_server_wait",
                   "fpsx_server_stats", "fpsx_server_submit_ex", "fpsx_server_wait_for",
                   "fpsx_server_cancel", "fpsx_server_counters"):
@@ -70,8 +70,10 @@ class Server:
     def submit(self, pc: np.ndarray, n_samples: int, start_idx: int = 0, priority: int = 0,
                deadline: Optional[float] = None) -> int:
         """Queue one 3-D cloud.  Higher `priority` is admitted first and may
-        pause lower-priority requests between chunks; `deadline` (seconds from
-        now) stops the request if it has not finished by then."""
+        pause lower-priority requests between chunks, or evict them to host
+        memory when HBM is short (they resume later with the same picks);
+        `deadline` (seconds from now) stops the request if it has not finished
+        by then."""
         pc = np.ascontiguousarray(pc, dtype=np.float32)
         rid = ctypes.c_uint64()
         rc = _lib().fpsx_server_submit_ex(self.handle,
@@ -118,12 +120,12 @@ class Server:
         z = [ctypes.c_size_t() for _ in range(3)]
         rc = _lib().fpsx_server_stats(self.handle, *[ctypes.byref(x) for x in u],
                                       ctypes.byref(busy), *[ctypes.byref(x) for x in z])
-        c = [ctypes.c_uint64() for _ in range(3)]
+        c = [ctypes.c_uint64() for _ in range(4)]
         _lib().fpsx_server_counters(self.handle, *[ctypes.byref(x) for x in c])
         return dict(launches=u[0].value, completed=u[1].value, samples=u[2].value,
                     busy_s=busy.value, hbm_in_use=z[0].value, pending=z[1].value,
                     active=z[2].value, cancelled=c[0].value, expired=c[1].value,
-                    paused=c[2].value, error=rc)
+                    paused=c[2].value, evicted=c[3].value, error=rc)
 
     def close(self) -> None:
         if self.handle:

@@ -332,7 +332,11 @@ void fpsx_server_destroy(fpsx_server *s);
  *   paused between two chunks -- its min-distances stay in its HBM range --
  *   and resumes, bit-identically, once the grid has room.  A request is
  *   admitted while the requests of its priority or higher leave room in the
- *   grid.  Paused requests keep their HBM (no eviction to host memory).
+ *   grid.  When its HBM is missing, lower-priority requests are evicted at
+ *   their next chunk boundary: each one's whole device range is copied to
+ *   host memory and freed, and it waits in the queue (ahead of the requests
+ *   of its priority that never ran) to be copied back and resumed where it
+ *   stopped, with the same picks.
  *   deadline_s > 0: seconds after submission by which the request must have
  *   finished; past it, a request not yet in its last chunk is stopped at the
  *   next chunk boundary (or dropped from the queue) and its wait returns
@@ -347,7 +351,8 @@ void fpsx_server_destroy(fpsx_server *s);
  *   none, 0: poll).  FPSX_EPENDING when the limit passes first; the request
  *   stays registered and can be waited for or cancelled again.
  * fpsx_server_counters: requests cancelled, requests stopped by their
- *   deadline, and (request, launch) pairs in which a request was paused.
+ *   deadline, (request, launch) pairs in which a request was paused, and
+ *   evictions to host memory.
  */
 int fpsx_server_submit_ex(fpsx_server *s, const float *points, size_t n_points, size_t dim,
                           size_t n_samples, uint64_t start_idx, int priority, double deadline_s,
@@ -356,7 +361,7 @@ int fpsx_server_cancel(fpsx_server *s, uint64_t request_id, int *cancelled);
 int fpsx_server_wait_for(fpsx_server *s, uint64_t request_id, double timeout_s, uint64_t *out,
                          double *latency_s);
 int fpsx_server_counters(fpsx_server *s, uint64_t *cancelled, uint64_t *expired,
-                         uint64_t *paused);
+                         uint64_t *paused, uint64_t *evicted);
 
 /*
  * Bucket FPS entry with the exact signature and return codes of the

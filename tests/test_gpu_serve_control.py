@@ -1,6 +1,7 @@
 """Request control of the serving loop on the GPU: cancellation (queued and
-running), deadlines (queued and running), waits with a time limit, and
-priorities that pause lower-priority requests between chunks.
+running), deadlines (queued and running), waits with a time limit,
+priorities that pause lower-priority requests between chunks, and eviction of
+lower-priority requests to host memory when HBM is short.
 
 What must hold: a request that is stopped ends with its own error and frees
 its HBM range; every request that is not stopped -- including ones paused
@@ -150,3 +151,33 @@ def test_priority_pauses_lower_requests(gpu):
     assert st["paused"] > 0
     # The high-priority request did not wait for the queue of low ones.
     assert done_hi < done_low[-1]
+
+
+@pytest.mark.skipif(not FL.available("gauss1m"), reason="full-length fixture missing")
+def test_eviction_under_hbm_pressure(gpu):
+    """A budget that holds the 2^20-point low-priority request or the
+    high-priority one, not both: the high one evicts the low one at a chunk
+    boundary (its whole range copied to host memory), runs, and the low one is
+    copied back and resumes.  Both return the reference's picks."""
+    from fpsample_amd.serve import Server
+
+    d = FL.load("gauss1m")
+    pc = bench.gen_gauss_mixture(1 << 20)
+    assert FL.sha256(pc) == d["cloud_sha256"]
+    m = int(d["m"])
+    hi_pc = cloud(9, 600000)
+    # 48 MiB: the low request needs ~33.7 MB, the high one ~19.3 MB.
+    with Server(hbm_budget_bytes=48 << 20, chunk_passes=8) as srv:
+        t_low = time.monotonic()
+        low = srv.submit(pc, m, 0, priority=0)
+        wait_until(lambda: srv.stats()["launches"] >= 3)
+        t_hi = time.monotonic()
+        hi = srv.submit(hi_pc, 2000, 6, priority=3)
+        got_hi, lat_hi = srv.wait(hi)
+        got_low, lat_low = srv.wait(low)
+        st = srv.stats()
+    log(f"evict: high {lat_hi:.4f} s, low {lat_low:.4f} s, stats {st}")
+    assert st["evicted"] >= 1 and st["error"] == 0 and st["hbm_in_use"] == 0
+    assert t_hi + lat_hi < t_low + lat_low
+    np.testing.assert_array_equal(got_low, d["out"])
+    np.testing.assert_array_equal(got_hi, O.fps(hi_pc, 2000, 6))
