@@ -191,12 +191,13 @@ __global__ void __launch_bounds__(B) fps_pruned(PrunedParams a) {
 // computed again before it can be used).  The grid must satisfy 2 <= G <= 64
 // (one polling wave evaluates all pairs).
 // ---------------------------------------------------------------------------
-template <int D, int B, int PPT, int TIE>
+template <int D, int B, int PPT, int TIE, int E>
 __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   constexpr int W = B / 64;
   constexpr int NW = 2 + D;      // value, original index, coordinates
   constexpr int NP = 2 * NW;     // words published per workgroup: A, then A'
-  constexpr int PER_WG = B * PPT;
+  constexpr int R = PPT + E;     // rows per wave: PPT in registers, E in LDS
+  constexpr int PER_WG = B * R;
   __shared__ int s_top[2][NW + 1][W];  // wave bests + "changed" flags, by exchange parity
   __shared__ int s_la[2][NW][W];       // wave lookahead bests
   __shared__ int s_res[2][NP + 2];     // p1, p2, two-pick flag, all-arrived flag
@@ -205,6 +206,8 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   __shared__ float s_wbox[2 * D][W];   // this workgroup's wave boxes
   __shared__ int s_ok;
   __shared__ uint32_t s_perm[PER_WG];
+  // Rows PPT .. R-1 of every wave (E > 0): coordinates and min-distance.
+  __shared__ float s_ex[E > 0 ? E : 1][D + 1][E > 0 ? W : 1][64];
 
   const FpsParams &f = a.fps;
   const int tid = threadIdx.x;
@@ -213,7 +216,7 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   const uint32_t G = gridDim.x;
   const uint32_t g = blockIdx.x;
   const uint32_t base = g * (uint32_t)PER_WG;
-  const uint32_t wbase = base + (uint32_t)(wave * 64 * PPT);
+  const uint32_t wbase = base + (uint32_t)(wave * 64 * R);
   const bool recorder = (g == 0 && tid >= B - 64);
 
   for (uint32_t q = tid; q < PER_WG; q += B)
@@ -243,18 +246,62 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
     }
     dm[k] = live ? f.init : -1.0f;
   }
+  if constexpr (E > 0) {
+#pragma unroll
+    for (int x = 0; x < E; ++x) {
+      const uint32_t i = wbase + (uint32_t)((PPT + x) * 64 + lane);
+      const bool live = i < f.n;
+      const uint32_t ic = live ? i : f.n - 1u;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const float v = a.sorted[(size_t)j * a.sorted_ld + ic];
+        s_ex[x][j][wave][lane] = live ? v : 0.0f;
+        lo[j] = live ? fminf(lo[j], v) : lo[j];
+        hi[j] = live ? fmaxf(hi[j], v) : hi[j];
+      }
+      s_ex[x][D][wave][lane] = live ? f.init : -1.0f;
+    }
+  }
 #pragma unroll
   for (int j = 0; j < D; ++j) {
     lo[j] = wave_min_f(lo[j]);
     hi[j] = wave_max_f(hi[j]);
   }
-  lds_barrier();  // s_perm complete
+  lds_barrier();  // s_perm (and the LDS rows) complete
 
-  // Best (value, original index, coordinates) of this wave over val(k).
-  // Each lane carries its best slot's coordinates through the scan (static
-  // indices only: selecting c[slot] afterwards by a uniform slot number is
-  // turned by the compiler into a dynamic index, which puts c in scratch).
-  auto wave_best = [&](auto val, int &ov, int &oi, float(&oc)[D]) {
+  // LDS rows (E > 0): min-distances under pivot q1 (when u1) and q2 (when u2).
+  auto update_ex = [&](bool u1, const float(&q1)[D], bool u2, const float(&q2)[D]) {
+    if constexpr (E > 0) {
+#pragma unroll
+      for (int x = 0; x < E; ++x) {
+        float ce[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) ce[j] = s_ex[x][j][wave][lane];
+        float d = s_ex[x][D][wave][lane];
+        if (u1) d = min_keep(sqdist<D>(ce, q1), d);
+        if (u2) d = min_keep(sqdist<D>(ce, q2), d);
+        s_ex[x][D][wave][lane] = d;
+      }
+    }
+  };
+
+  // Best (value, original index, coordinates) of this wave over val(k) for
+  // the register rows and xval(x, coordinates) for the LDS rows (after them
+  // in position order).  Each lane carries its best slot's coordinates
+  // through the scan (static indices only: selecting c[slot] afterwards by a
+  // uniform slot number is turned by the compiler into a dynamic index,
+  // which puts c in scratch).
+  auto wave_best = [&](auto val, auto xval, int &ov, int &oi, float(&oc)[D]) {
+    float xc[E > 0 ? E : 1][D];
+    float xv[E > 0 ? E : 1];
+    if constexpr (E > 0) {
+#pragma unroll
+      for (int x = 0; x < E; ++x) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) xc[x][j] = s_ex[x][j][wave][lane];
+        xv[x] = xval(x, xc[x]);
+      }
+    }
     int bv = INT_MIN;
     int bk = 0;
     float bc[D];
@@ -269,6 +316,17 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
 #pragma unroll
       for (int j = 0; j < D; ++j) bc[j] = take ? c[k][j] : bc[j];
     }
+    if constexpr (E > 0) {
+#pragma unroll
+      for (int x = 0; x < E; ++x) {
+        const int b = __float_as_int(xv[x]);
+        const bool take = (TIE == kTieHighest) ? (b >= bv) : (b > bv);
+        bv = take ? b : bv;
+        bk = take ? PPT + x : bk;
+#pragma unroll
+        for (int j = 0; j < D; ++j) bc[j] = take ? xc[x][j] : bc[j];
+      }
+    }
     const uint32_t bpos = (uint32_t)(bk * 64 + lane);
     int wv;
     const int wl = wave_argmax_lane<TIE>(bv, bpos, wv);
@@ -276,8 +334,10 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
 #pragma unroll
     for (int j = 0; j < D; ++j) oc[j] = readlane_f(bc[j], wl);
     ov = wv;
-    oi = (int)s_perm[wave * 64 * PPT + wpos];
+    oi = (int)s_perm[wave * 64 * R + wpos];
   };
+  // The LDS rows' stored min-distances, for wave_best after an update.
+  auto xstored = [&](int x, const float(&)[D]) { return E > 0 ? s_ex[x][D][wave][lane] : 0.0f; };
 
   float p1[D], p2[D];
   uint32_t i1 = load_forced<D>(f, 0, p1);
@@ -298,7 +358,8 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
     if (!cached || box_bound<D>(p1, lo, hi) < __int_as_float(cv)) {
 #pragma unroll
       for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p1), dm[k]);
-      wave_best([&](int k) { return dm[k]; }, cv, ci, cc);
+      update_ex(true, p1, false, p1);
+      wave_best([&](int k) { return dm[k]; }, xstored, cv, ci, cc);
       cached = true;
     }
     i1 = load_forced<D>(f, s, p1);
@@ -405,7 +466,8 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
       for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p2), dm[k]);
     }
     if (need1 || need2) {
-      wave_best([&](int k) { return dm[k]; }, cv, ci, cc);
+      update_ex(need1, p1, need2, p2);
+      wave_best([&](int k) { return dm[k]; }, xstored, cv, ci, cc);
       cached = true;
     }
 
@@ -463,7 +525,11 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
 #pragma unroll
       for (int j = 0; j < D; ++j) lc[j] = cc[j];
       if ((sweepers >> wave) & 1ull)
-        wave_best([&](int k) { return min_keep(sqdist<D>(c[k], pa), dm[k]); }, lv, li, lc);
+        wave_best([&](int k) { return min_keep(sqdist<D>(c[k], pa), dm[k]); },
+                  [&](int x, const float(&ce)[D]) {
+                    return E > 0 ? min_keep(sqdist<D>(ce, pa), s_ex[x][D][wave][lane]) : 0.0f;
+                  },
+                  lv, li, lc);
       if (lane == 0) {
         s_la[par][0][wave] = lv;
         s_la[par][1][wave] = li;
@@ -1372,12 +1438,14 @@ const KernelEntry *pruned_table(size_t *count) {
   return kPruned;
 }
 
-#define FPSX_PAIR(D, B, P)                                                          \
-  {D, B, P, kTieHighest, (const void *)&fps_pruned_pair<D, B, P, kTieHighest>}, \
-  {D, B, P, kTieLowest, (const void *)&fps_pruned_pair<D, B, P, kTieLowest>}
+// (points per thread P = registers PPT + LDS E)
+#define FPSX_PAIR(D, B, PPT, E)                                                                 \
+  {D, B, PPT + E, kTieHighest, (const void *)&fps_pruned_pair<D, B, PPT, kTieHighest, E>}, \
+  {D, B, PPT + E, kTieLowest, (const void *)&fps_pruned_pair<D, B, PPT, kTieLowest, E>}
 
 static const KernelEntry kPair[] = {
-    FPSX_PAIR(3, 1024, 16), FPSX_PAIR(3, 1024, 8), FPSX_PAIR(3, 1024, 4),
+    FPSX_PAIR(3, 1024, 16, 0), FPSX_PAIR(3, 1024, 8, 0), FPSX_PAIR(3, 1024, 4, 0),
+    FPSX_PAIR(3, 1024, 16, 1),
 };
 #undef FPSX_PAIR
 

@@ -324,14 +324,16 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
   if (prune && n > 16384 && m >= 1024) {
     // Paired form first (half the exchanges) where its grid limit allows.
     if (!(npair && npair[0] == '1')) {
-      static const int kPa[][2] = {{1024, 4}, {1024, 8}, {1024, 16}};
+      // 1024 x 17: 16 points per thread in registers and one in LDS, so 64
+      // workgroups hold up to 1,114,112 points without the wider exchange.
+      static const int kPa[][2] = {{1024, 4}, {1024, 8}, {1024, 16}, {1024, 17}};
       for (auto &sh : kPa) {
         const KernelEntry *k = find_pair(3, sh[0], sh[1], tie);
         if (k && plan_resident_shape(c, n, k, p, PlanKind::Pair) == FPSX_OK && p.grid >= 2 &&
             p.grid <= kPairMaxGrid)
           return FPSX_OK;
       }
-      // Beyond 64 workgroups (more than 1 M points): 1024 x 16 with two or
+      // Beyond 64 workgroups (more than 1,114,112 points): 1024 x 16 with two or
       // four polling waves, up to 256 workgroups.
       for (int pw : {2, 4}) {
         const KernelEntry *k = pair_wide_kernel(1024, 16, tie, pw);

@@ -76,7 +76,9 @@ void *fpsx_stream(void);
  * 2 = register-resident with spatial order and per-wave pruning,
  * 4 = HBM-streaming with spatial order and per-block pruning,
  * 5 = kind 2 settling up to two picks per cross-workgroup exchange (2..256
- * workgroups; beyond 64, 1024 x 16 with 2 or 4 waves polling the exchange);
+ * workgroups; ppt 17 = 1024 x 16 points in registers plus one per thread in
+ * LDS, taken for 2^20 < n_points <= 1,114,112; beyond 64 workgroups, 1024 x 16
+ * with 2 or 4 waves polling the exchange);
  * threads per workgroup, points per thread (0 when streaming), workgroups,
  * points per workgroup.  FPSX_PLAN in the environment forces a plan:
  * "resident:B:PPT", "pruned:B:PPT", "pair:B:PPT", "pairwPW:B:PPT" (the paired

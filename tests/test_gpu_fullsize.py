@@ -60,13 +60,18 @@ def test_gauss1m_every_pick(gpu, monkeypatch, gauss1m, env):
     check_single(gpu, monkeypatch, "gauss1m", gauss1m, env)
 
 
-@pytest.mark.parametrize("n,name,env", [((1 << 20) + 1, "gauss1m1", {}), (1 << 21, "gauss2m", {}),
+@pytest.mark.parametrize("n,name,env", [((1 << 20) + 1, "gauss1m1", {}),
+                                        ((1 << 20) + 1, "gauss1m1", {"FPSX_PLAN": "pairw2:1024:16"}),
+                                        (1 << 21, "gauss2m", {}),
                                         (1 << 21, "gauss2m", {"FPSX_PLAN": "pairw4:1024:16"}),
                                         (1 << 21, "gauss2m", {"FPSX_NO_PAIR": "1"})],
-                         ids=["gauss1m1-auto", "gauss2m-auto", "gauss2m-pairw4", "gauss2m-no_pair"])
+                         ids=["gauss1m1-auto", "gauss1m1-pairw2", "gauss2m-auto", "gauss2m-pairw4",
+                              "gauss2m-no_pair"])
 def test_gauss_beyond_1m_every_pick(gpu, monkeypatch, n, name, env):
-    # The planner's paired kernel with two polling waves (65 and 128
-    # workgroups), four waves forced, and the one-pick pruned kernel.
+    # The planner's choices (2^20 + 1 points: 61 workgroups of 1024 x 17, one
+    # point per thread in LDS; 2^21: two polling waves over 128 workgroups),
+    # the two-wave form forced at 65 workgroups, four waves forced, and the
+    # one-pick pruned kernel.
     check_single(gpu, monkeypatch, name, bench.gen_gauss_mixture(n), env)
 
 

@@ -47,7 +47,8 @@ def gauss_cloud(n, seed):
     return (centers[lab] + 0.02 * rng.standard_normal((n, 3))).astype(np.float32)
 
 
-PAIR_SHAPES = ["pair:1024:16", "pair:1024:8", "pair:1024:4"]
+# 1024 x 17: 16 points per thread in registers and one in LDS.
+PAIR_SHAPES = ["pair:1024:16", "pair:1024:8", "pair:1024:4", "pair:1024:17"]
 
 
 @pytest.mark.parametrize("shape", PAIR_SHAPES)
@@ -114,6 +115,37 @@ def test_pair_lowest_ties_bucket_entry(gpu, plan_env):
     rc, out = gpu.kdline(lat, 1500, 123, 7)
     assert rc == 0, gpu.last_error()
     np.testing.assert_array_equal(out, O.kdline_quickfps(lat, 1500, 7, 123))
+
+
+def test_pair_lds_row_lowest_ties_and_specials(gpu, plan_env):
+    # The LDS row of 1024 x 17: lowest-position ties through the bucket entry
+    # on a lattice, and NaN / inf / huge coordinates.
+    plan_env("pair:1024:17")
+    rng = np.random.default_rng(46)
+    lat = rng.integers(0, 40, (100000, 3)).astype(np.float32)
+    log("bucket entry, lattice, pair:1024:17")
+    rc, out = gpu.kdline(lat, 1500, 123, 7)
+    assert rc == 0, gpu.last_error()
+    np.testing.assert_array_equal(out, O.kdline_quickfps(lat, 1500, 7, 123))
+    pc = (rng.random((60000, 3)) * 4e19).astype(np.float32)
+    pc[[5, 17500, 40000]] = np.nan
+    pc[9] = np.inf
+    log("special values, pair:1024:17")
+    np.testing.assert_array_equal(run_fps(gpu, pc, 700, [0, 3]), O.fps(pc, 700, [0, 3]))
+
+
+@pytest.mark.parametrize("n", [(1 << 20) + 1, 64 * 17408])
+def test_pair_lds_row_planner_sizes(gpu, plan_env, n):
+    # Just beyond 2^20 points and the largest cloud 64 workgroups of 1024 x 17
+    # hold: the planner keeps the one-wave paired kernel, oracle-compared on a
+    # prefix (clusters and 10 % exact duplicates, a forced two-pick prefix).
+    plan_env(None)
+    pc = gauss_cloud(n, 12)
+    pc[::10] = pc[1::10][: len(pc[::10])]
+    info = gpu.plan_info(n, 3, m=16384)
+    log(f"n={n} planner(16384)={info}")
+    assert info["kind"] == 5 and info["ppt"] == 17 and info["grid"] <= 64, info
+    np.testing.assert_array_equal(run_fps(gpu, pc, 16384, [3, 0])[:600], O.fps(pc, 600, [3, 0]))
 
 
 def test_pair_full_size_matches_one_pick_kernel(gpu, plan_env):
