@@ -391,6 +391,65 @@ def _reference_check(name: str, picks_list) -> object:
     return all(FL.picks_sha256(p) == str(h) for p, h in zip(picks_list, d["picks_sha256"]))
 
 
+def _ref_fps_worker(item):
+    """Process-pool task: the reference's fps_sampling on one cloud (seconds)."""
+    pc, m = item
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+
+    ref = O.load_reference()
+    run = (lambda: ref.fps_sampling(pc, m, 0)) if ref is not None else (lambda: O.fps(pc, m, 0))
+    t0 = time.perf_counter()
+    run()
+    return time.perf_counter() - t0
+
+
+def batch_cpu_baseline(clouds, ms, target_s: float) -> dict:
+    """The 256 ragged clouds on the host: the reference (oracle/_ref, else the
+    oracle's C restatement) single-threaded on one pinned core over whole
+    clouds in order until target_s has passed, and every cloud over a process
+    pool of up to 16 workers (the GPU host's share), each cloud one task."""
+    from oracle import oracle as O
+
+    ref = O.load_reference()
+    kind = "reference" if ref is not None else "port"
+    done_m, done_t, k = 0, 0.0, 0
+    with _pinned() as pin:
+        for pc, m in zip(clouds, ms):
+            t0 = time.perf_counter()
+            if ref is not None:
+                ref.fps_sampling(pc, m, 0)
+            else:
+                O.fps(pc, m, 0)
+            done_t += time.perf_counter() - t0
+            done_m += m
+            k += 1
+            if done_t >= target_s:
+                break
+    total_m = sum(ms)
+    out = {"value": round(done_m / done_t, 3), "unit": "samples/s", "cores": 1, "kind": kind,
+           "sample": f"the first {k} of the {len(clouds)} clouds ({100.0 * done_m / total_m:.1f} % of the "
+                     f"samples), {done_t:.2f} s single-threaded on core {pin.core} of a {cpu_model()}"}
+    workers = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    try:
+        import multiprocessing as mp
+        from concurrent.futures import ProcessPoolExecutor
+
+        # spawned workers: fresh interpreters that never touch the GPU
+        with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+            list(ex.map(_ref_fps_worker, [(clouds[0][:64], 8)] * workers))  # start every worker
+            order = sorted(range(len(clouds)), key=lambda i: -clouds[i].shape[0] * ms[i])
+            t0 = time.perf_counter()
+            list(ex.map(_ref_fps_worker, [(clouds[i], ms[i]) for i in order]))
+            wall = time.perf_counter() - t0
+        out["pool"] = {"value": round(total_m / wall, 3), "unit": "samples/s", "cores": workers,
+                       "sample": f"all {len(clouds)} clouds, longest first, {wall:.2f} s over {workers} "
+                                 f"worker processes"}
+    except Exception as exc:  # a host without process support: single-core figure only
+        out["pool"] = {"error": str(exc)}
+    return out
+
+
 def run_batch(args) -> dict:
     """Batched ragged clouds: one persistent launch (HIP graph) vs one launch per cloud."""
     from fpsample_amd import device as D
@@ -434,6 +493,7 @@ def run_batch(args) -> dict:
     single = [o.numpy() for o in outs]
     same = all(np.array_equal(a, o) for a, o in zip(batched, single))
     b.close()
+    cpu = batch_cpu_baseline(clouds, ms, min(10.0, args.cpu_seconds)) if args.cpu_seconds > 0 else None
     return {
         "metric": "samples/s (256 ragged clouds, one persistent launch)",
         "value": round(total_m / wall_b, 3),
@@ -450,6 +510,7 @@ def run_batch(args) -> dict:
         "target_speedup": 5.0,
         "identical_output": bool(same),
         "identical_to_reference": _reference_check("batch256", batched),
+        "cpu_baseline": cpu,
     }
 
 
