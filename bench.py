@@ -119,6 +119,14 @@ WORKLOADS = {
     "gauss1m": dict(gen=gen_gauss_mixture, m=16384,
                     desc="one cloud N=1048576 D=3 f32, 64 Gaussian clusters sigma 0.02, "
                          "M=16384, start_idx=0"),
+    # just beyond the 1 M-point cloud (one point per thread in LDS) and twice it
+    # (the wide paired kernel, 128 workgroups): the same generator, M=16384
+    "gauss1m1": dict(gen=lambda: gen_gauss_mixture((1 << 20) + 1), m=16384,
+                     desc="one cloud N=1048577 D=3 f32, 64 Gaussian clusters sigma 0.02, "
+                          "M=16384, start_idx=0"),
+    "gauss2m": dict(gen=lambda: gen_gauss_mixture(1 << 21), m=16384,
+                    desc="one cloud N=2097152 D=3 f32, 64 Gaussian clusters sigma 0.02, "
+                         "M=16384, start_idx=0"),
     "uniform4k": dict(gen=gen_uniform_4096, m=1024,
                       desc="one cloud N=4096 D=3 f32 uniform [0,1)^3 seed 42, M=1024, start_idx=0"),
     "lattice16m": dict(gen=gen_lattice_dups, m=4096,
@@ -735,7 +743,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": None,
     }
-    if not npdu and args.workload in ("gauss1m", "lattice16m"):
+    if not npdu and args.workload in ("gauss1m", "gauss1m1", "gauss2m", "lattice16m"):
         ref = None
         try:
             from tests.golden import full_length as FL
