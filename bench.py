@@ -531,7 +531,7 @@ def run_serve(args) -> dict:
 
     reqs = gen_serving_stream()
     total_m = sum(m for _, m in reqs)
-    with Server(hbm_budget_bytes=16 << 30, chunk_passes=256) as srv:
+    with Server(hbm_budget_bytes=16 << 30, chunk_passes=args.serve_chunk) as srv:
         srv.wait(srv.submit(reqs[0][0], reqs[0][1], 0))  # warm-up
         t0 = time.perf_counter()
         ids = [srv.submit(pc, m, 0) for pc, m in reqs]
@@ -573,7 +573,7 @@ def run_serve(args) -> dict:
         "metric": "p50/p99 latency, mixed FPS stream",
         "config": {"workload": "96 requests, N log-uniform [2^10, 2^20], uniform cube, "
                                "M = N*U{1/8,1/4,1/2}, start_idx=0, HBM budget 16 GiB, "
-                               "256-pass chunks, batch kernel 1024 x 16",
+                               f"{args.serve_chunk}-pass chunks, batch kernel 1024 x 16",
                    "total_samples": total_m, "qps": round(qps, 3)},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -651,6 +651,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--extras", default="on", choices=["on", "off"],
                     help="gauss1m: also run the serve and batch256 legs into the same line")
+    ap.add_argument("--serve-chunk", type=int, default=256,
+                    help="serve: passes per request per launch of the serving loop")
     ap.add_argument("--autotune", default="off", choices=["on", "off"],
                     help="time every plan on this cloud first (off: the planner's choice)")
     args = ap.parse_args()
