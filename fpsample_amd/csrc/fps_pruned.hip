@@ -198,9 +198,13 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   constexpr int NP = 2 * NW;     // words published per workgroup: A, then A'
   constexpr int R = PPT + E;     // rows per wave: PPT in registers, E in LDS
   constexpr int PER_WG = B * R;
-  __shared__ int s_top[2][NW + 1][W];  // wave bests + "changed" flags, by exchange parity
-  __shared__ int s_la[2][NW][W];       // wave lookahead bests
-  __shared__ int s_res[2][NP + 2];     // p1, p2, two-pick flag, all-arrived flag
+  // LDS hand-offs as 16-byte-aligned records, moved with 128-bit accesses:
+  // per wave its best and "changed" flag, its lookahead best; the round's
+  // result (p1, p2, two-pick flag, all-arrived flag).  By exchange parity.
+  static_assert(D == 3, "the records below hold three coordinates");
+  __shared__ int4 s_top[2][W][2];
+  __shared__ int4 s_la[2][W][2];
+  __shared__ int4 s_res[2][4];
   __shared__ int s_mu[2][2];           // M of the round of this parity
   __shared__ float s_gbox[2 * D][64];  // every workgroup's bounding box
   __shared__ float s_wbox[2 * D][W];   // this workgroup's wave boxes
@@ -474,17 +478,21 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
     ++e;
     const int par = (int)(e & 1u);
     if (lane == 0) {
-      s_top[par][0][wave] = cv;
-      s_top[par][1][wave] = ci;
-#pragma unroll
-      for (int j = 0; j < D; ++j) s_top[par][2 + j][wave] = __float_as_int(cc[j]);
-      s_top[par][NW][wave] = (need1 || need2) ? 1 : 0;
+      s_top[par][wave][0] = make_int4(cv, ci, __float_as_int(cc[0]), __float_as_int(cc[1]));
+      s_top[par][wave][1] = make_int4(__float_as_int(cc[2]), (need1 || need2) ? 1 : 0, 0, 0);
     }
     lds_barrier();
     int tw[NW + 1];
-#pragma unroll
-    for (int k = 0; k <= NW; ++k) tw[k] = (lane < W) ? s_top[par][k][lane] : 0;
-    if (lane >= W) tw[0] = INT_MIN;
+    {
+      const int4 t0 = (lane < W) ? s_top[par][lane][0] : make_int4(INT_MIN, 0, 0, 0);
+      const int4 t1 = (lane < W) ? s_top[par][lane][1] : make_int4(0, 0, 0, 0);
+      tw[0] = t0.x;
+      tw[1] = t0.y;
+      tw[2] = t0.z;
+      tw[3] = t0.w;
+      tw[4] = t1.x;
+      tw[5] = t1.y;
+    }
     // M of the previous round (written by wave 0 before this barrier).
     mu_v = s_mu[par ^ 1][0];
     mu_i = (uint32_t)s_mu[par ^ 1][1];
@@ -531,16 +539,20 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
                   },
                   lv, li, lc);
       if (lane == 0) {
-        s_la[par][0][wave] = lv;
-        s_la[par][1][wave] = li;
-#pragma unroll
-        for (int j = 0; j < D; ++j) s_la[par][2 + j][wave] = __float_as_int(lc[j]);
+        s_la[par][wave][0] = make_int4(lv, li, __float_as_int(lc[0]), __float_as_int(lc[1]));
+        s_la[par][wave][1] = make_int4(__float_as_int(lc[2]), 0, 0, 0);
       }
       lds_barrier();
       int lw[NW];
-#pragma unroll
-      for (int k = 0; k < NW; ++k) lw[k] = (lane < W) ? s_la[par][k][lane] : 0;
-      if (lane >= W) lw[0] = INT_MIN;
+      {
+        const int4 t0 = (lane < W) ? s_la[par][lane][0] : make_int4(INT_MIN, 0, 0, 0);
+        const int t1 = (lane < W) ? s_la[par][lane][1].x : 0;
+        lw[0] = t0.x;
+        lw[1] = t0.y;
+        lw[2] = t0.z;
+        lw[3] = t0.w;
+        lw[4] = t1;
+      }
       int lgv;
       const int wl2 = wave_argmax_lane<TIE>(lw[0], (uint32_t)lw[1], lgv);
 #pragma unroll
@@ -639,20 +651,28 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
         r2[j] = readlane_i(cw, l2);
       }
       if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NW; ++k) s_res[par][k] = r1[k];
-        s_res[par][NW] = v2;
-        s_res[par][NW + 1] = (int)i2;
-#pragma unroll
-        for (int j = 0; j < D; ++j) s_res[par][NW + 2 + j] = r2[j];
-        s_res[par][NP] = two ? 1 : 0;
-        s_res[par][NP + 1] = all_ok ? 1 : 0;
+        s_res[par][0] = make_int4(r1[0], r1[1], r1[2], r1[3]);
+        s_res[par][1] = make_int4(r1[4], v2, (int)i2, r2[0]);
+        s_res[par][2] = make_int4(r2[1], r2[2], two ? 1 : 0, all_ok ? 1 : 0);
       }
     }
     lds_barrier();
     int res[NP + 2];
-#pragma unroll
-    for (int k = 0; k < NP + 2; ++k) res[k] = s_res[par][k];
+    {
+      const int4 q0 = s_res[par][0], q1 = s_res[par][1], q2 = s_res[par][2];
+      res[0] = q0.x;
+      res[1] = q0.y;
+      res[2] = q0.z;
+      res[3] = q0.w;
+      res[4] = q1.x;
+      res[5] = q1.y;
+      res[6] = q1.z;
+      res[7] = q1.w;
+      res[8] = q2.x;
+      res[9] = q2.y;
+      res[10] = q2.z;
+      res[11] = q2.w;
+    }
     if (res[NP + 1] == 0) {
       if (tid == 0) atomicOr(f.status, kStatusTimeout);
       return;
