@@ -205,13 +205,14 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   __shared__ int4 s_top[2][W][2];
   __shared__ int4 s_la[2][W][2];
   __shared__ int4 s_res[2][4];
-  __shared__ int s_mu[2][2];           // M of the round of this parity
+  __shared__ int2 s_mu[2];             // M of the round of this parity
   __shared__ float s_gbox[2 * D][64];  // every workgroup's bounding box
   __shared__ float s_wbox[2 * D][W];   // this workgroup's wave boxes
   __shared__ int s_ok;
   __shared__ uint32_t s_perm[PER_WG];
-  // Rows PPT .. R-1 of every wave (E > 0): coordinates and min-distance.
-  __shared__ float s_ex[E > 0 ? E : 1][D + 1][E > 0 ? W : 1][64];
+  // Rows PPT .. R-1 of every wave (E > 0): coordinates and min-distance of a
+  // slot in one 16-byte record (x, y, z, min-distance).
+  __shared__ float4 s_ex[E > 0 ? E : 1][E > 0 ? W : 1][64];
 
   const FpsParams &f = a.fps;
   const int tid = threadIdx.x;
@@ -256,14 +257,15 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
       const uint32_t i = wbase + (uint32_t)((PPT + x) * 64 + lane);
       const bool live = i < f.n;
       const uint32_t ic = live ? i : f.n - 1u;
+      float v[D];
 #pragma unroll
       for (int j = 0; j < D; ++j) {
-        const float v = a.sorted[(size_t)j * a.sorted_ld + ic];
-        s_ex[x][j][wave][lane] = live ? v : 0.0f;
-        lo[j] = live ? fminf(lo[j], v) : lo[j];
-        hi[j] = live ? fmaxf(hi[j], v) : hi[j];
+        v[j] = a.sorted[(size_t)j * a.sorted_ld + ic];
+        lo[j] = live ? fminf(lo[j], v[j]) : lo[j];
+        hi[j] = live ? fmaxf(hi[j], v[j]) : hi[j];
+        v[j] = live ? v[j] : 0.0f;
       }
-      s_ex[x][D][wave][lane] = live ? f.init : -1.0f;
+      s_ex[x][wave][lane] = make_float4(v[0], v[1], v[2], live ? f.init : -1.0f);
     }
   }
 #pragma unroll
@@ -278,20 +280,19 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
     if constexpr (E > 0) {
 #pragma unroll
       for (int x = 0; x < E; ++x) {
-        float ce[D];
-#pragma unroll
-        for (int j = 0; j < D; ++j) ce[j] = s_ex[x][j][wave][lane];
-        float d = s_ex[x][D][wave][lane];
+        const float4 r = s_ex[x][wave][lane];
+        const float ce[D] = {r.x, r.y, r.z};
+        float d = r.w;
         if (u1) d = min_keep(sqdist<D>(ce, q1), d);
         if (u2) d = min_keep(sqdist<D>(ce, q2), d);
-        s_ex[x][D][wave][lane] = d;
+        s_ex[x][wave][lane].w = d;
       }
     }
   };
 
   // Best (value, original index, coordinates) of this wave over val(k) for
-  // the register rows and xval(x, coordinates) for the LDS rows (after them
-  // in position order).  Each lane carries its best slot's coordinates
+  // the register rows and xval(x, coordinates, stored min-distance) for the
+  // LDS rows (after them in position order).  Each lane carries its best slot's coordinates
   // through the scan (static indices only: selecting c[slot] afterwards by a
   // uniform slot number is turned by the compiler into a dynamic index,
   // which puts c in scratch).
@@ -301,9 +302,11 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
     if constexpr (E > 0) {
 #pragma unroll
       for (int x = 0; x < E; ++x) {
-#pragma unroll
-        for (int j = 0; j < D; ++j) xc[x][j] = s_ex[x][j][wave][lane];
-        xv[x] = xval(x, xc[x]);
+        const float4 r = s_ex[x][wave][lane];
+        xc[x][0] = r.x;
+        xc[x][1] = r.y;
+        xc[x][2] = r.z;
+        xv[x] = xval(x, xc[x], r.w);
       }
     }
     int bv = INT_MIN;
@@ -341,7 +344,7 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
     oi = (int)s_perm[wave * 64 * R + wpos];
   };
   // The LDS rows' stored min-distances, for wave_best after an update.
-  auto xstored = [&](int x, const float(&)[D]) { return E > 0 ? s_ex[x][D][wave][lane] : 0.0f; };
+  auto xstored = [&](int, const float(&)[D], float dmx) { return dmx; };
 
   float p1[D], p2[D];
   uint32_t i1 = load_forced<D>(f, 0, p1);
@@ -387,10 +390,7 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
   const int K = min(min((int)f.replicas, 64 / NP), 4);
 
   // Bounding boxes of all workgroups: one exchange (number 1) of 2*D words.
-  if (tid < 2) {
-    s_mu[tid][0] = INT_MIN;
-    s_mu[tid][1] = 0;
-  }
+  if (tid < 2) s_mu[tid] = make_int2(INT_MIN, 0);
   if (lane == 0) {
 #pragma unroll
     for (int j = 0; j < D; ++j) {
@@ -494,8 +494,11 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
       tw[5] = t1.y;
     }
     // M of the previous round (written by wave 0 before this barrier).
-    mu_v = s_mu[par ^ 1][0];
-    mu_i = (uint32_t)s_mu[par ^ 1][1];
+    {
+      const int2 mu = s_mu[par ^ 1];
+      mu_v = mu.x;
+      mu_i = (uint32_t)mu.y;
+    }
     // The first round computes A everywhere: after a forced prefix a wave's
     // candidate is cached, but the workgroup's A has never been formed.
     const bool changed = e == 2u || __ballot(lane < W && tw[NW] != 0) != 0;
@@ -534,9 +537,7 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
       for (int j = 0; j < D; ++j) lc[j] = cc[j];
       if ((sweepers >> wave) & 1ull)
         wave_best([&](int k) { return min_keep(sqdist<D>(c[k], pa), dm[k]); },
-                  [&](int x, const float(&ce)[D]) {
-                    return E > 0 ? min_keep(sqdist<D>(ce, pa), s_ex[x][D][wave][lane]) : 0.0f;
-                  },
+                  [&](int, const float(&ce)[D], float dmx) { return min_keep(sqdist<D>(ce, pa), dmx); },
                   lv, li, lc);
       if (lane == 0) {
         s_la[par][wave][0] = make_int4(lv, li, __float_as_int(lc[0]), __float_as_int(lc[1]));
@@ -705,10 +706,7 @@ __global__ void __launch_bounds__(B) fps_pruned_pair(PrunedParams a) {
       const int ml = wave_argmax_lane<TIE>((keep && (uint32_t)lane != g) ? sv : INT_MIN,
                                            (uint32_t)si, mv);
       const int mi = readlane_i(si, ml);
-      if (lane == 0) {
-        s_mu[par][0] = mv;
-        s_mu[par][1] = mi;
-      }
+      if (lane == 0) s_mu[par] = make_int2(mv, mi);
     }
     if (recorder) {
       record_pick<D>(f, s, i1, p1);
