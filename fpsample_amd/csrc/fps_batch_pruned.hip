@@ -26,16 +26,26 @@
 
 namespace fpsx {
 
-template <int B, int PPT, int TIE>
+// VEC: the hand-offs as 16-byte records moved with 128-bit LDS accesses
+// (claim-loop batch launches: 29.15 -> 28.4 ms on the 256-cloud batch), or
+// word by word (the serving loop's one-slot 1024 x 16 kernel, where the
+// records cost one more spilled VGPR and 1.8 % of busy time).
+template <int B, int PPT, int TIE, bool VEC>
 struct PPShared {
   static constexpr int D = 3;
   static constexpr int W = B / 64;
   static constexpr int NW = 2 + D;
   static constexpr int NP = 2 * NW;
-  int top[2][NW + 1][W];   // wave bests + "changed" flags, by exchange parity
-  int la[2][NW][W];        // wave lookahead bests
-  int res[2][NP + 2];      // p1, p2, two-pick flag, all-arrived flag
-  int mu[2][2];            // M of the round of this parity
+  // Hand-offs by exchange parity: per wave its best and "changed" flag, its
+  // lookahead best; the round's result (p1, p2, two-pick flag, all-arrived
+  // flag); M.
+  int4 top4[VEC ? 2 : 1][VEC ? W : 1][2];
+  int4 la4[VEC ? 2 : 1][VEC ? W : 1][2];
+  int4 res4[VEC ? 2 : 1][4];
+  int top[VEC ? 1 : 2][VEC ? 1 : NW + 1][VEC ? 1 : W];
+  int la[VEC ? 1 : 2][VEC ? 1 : NW][VEC ? 1 : W];
+  int res[VEC ? 1 : 2][VEC ? 1 : NP + 2];
+  int2 mu[2];
   float gbox[2 * D][64];   // every member's bounding box
   float wbox[2 * D][W];    // this member's wave boxes
   int ok;
@@ -45,9 +55,9 @@ struct PPShared {
 
 // One member's share of job J (member r of J.team).  Returns false when an
 // exchange wait ran out.
-template <int B, int PPT, int TIE>
+template <int B, int PPT, int TIE, bool VEC>
 __device__ __forceinline__ bool run_job_pp(const BatchPPParams &ap, uint32_t r,
-                                           PPShared<B, PPT, TIE> &sh) {
+                                           PPShared<B, PPT, TIE, VEC> &sh) {
   constexpr int D = 3;
   constexpr int W = B / 64;
   constexpr int NW = 2 + D;
@@ -179,8 +189,7 @@ __device__ __forceinline__ bool run_job_pp(const BatchPPParams &ap, uint32_t r,
   constexpr int K = (int)R < 64 / NP ? (int)R : 64 / NP;  // replicas a pair's words allow
 
   if (tid < 2) {
-    sh.mu[tid][0] = INT_MIN;
-    sh.mu[tid][1] = 0;
+    sh.mu[tid] = make_int2(INT_MIN, 0);
   }
   if (lane == 0) {
 #pragma unroll
@@ -274,20 +283,39 @@ __device__ __forceinline__ bool run_job_pp(const BatchPPParams &ap, uint32_t r,
 
     ++e;
     const int par = (int)(e & 1u);
-    if (lane == 0) {
-      sh.top[par][0][wave] = cv;
-      sh.top[par][1][wave] = ci;
-#pragma unroll
-      for (int j = 0; j < D; ++j) sh.top[par][2 + j][wave] = __float_as_int(cc[j]);
-      sh.top[par][NW][wave] = (need1 || need2) ? 1 : 0;
-    }
-    lds_barrier();
     int tw[NW + 1];
+    if constexpr (VEC) {
+      if (lane == 0) {
+        sh.top4[par][wave][0] = make_int4(cv, ci, __float_as_int(cc[0]), __float_as_int(cc[1]));
+        sh.top4[par][wave][1] = make_int4(__float_as_int(cc[2]), (need1 || need2) ? 1 : 0, 0, 0);
+      }
+      lds_barrier();
+      const int4 t0 = (lane < W) ? sh.top4[par][lane][0] : make_int4(INT_MIN, 0, 0, 0);
+      const int4 t1 = (lane < W) ? sh.top4[par][lane][1] : make_int4(0, 0, 0, 0);
+      tw[0] = t0.x;
+      tw[1] = t0.y;
+      tw[2] = t0.z;
+      tw[3] = t0.w;
+      tw[4] = t1.x;
+      tw[5] = t1.y;
+    } else {
+      if (lane == 0) {
+        sh.top[par][0][wave] = cv;
+        sh.top[par][1][wave] = ci;
 #pragma unroll
-    for (int k = 0; k <= NW; ++k) tw[k] = (lane < W) ? sh.top[par][k][lane] : 0;
-    if (lane >= W) tw[0] = INT_MIN;
-    mu_v = sh.mu[par ^ 1][0];
-    mu_i = (uint32_t)sh.mu[par ^ 1][1];
+        for (int j = 0; j < D; ++j) sh.top[par][2 + j][wave] = __float_as_int(cc[j]);
+        sh.top[par][NW][wave] = (need1 || need2) ? 1 : 0;
+      }
+      lds_barrier();
+#pragma unroll
+      for (int k = 0; k <= NW; ++k) tw[k] = (lane < W) ? sh.top[par][k][lane] : 0;
+      if (lane >= W) tw[0] = INT_MIN;
+    }
+    {
+      const int2 mu = sh.mu[par ^ 1];
+      mu_v = mu.x;
+      mu_i = (uint32_t)mu.y;
+    }
     // The chunk's first round forms A everywhere.
     const bool changed = e == 2u || __ballot(lane < W && tw[NW] != 0) != 0;
     if (changed) {
@@ -321,17 +349,32 @@ __device__ __forceinline__ bool run_job_pp(const BatchPPParams &ap, uint32_t r,
       for (int j = 0; j < D; ++j) lc[j] = cc[j];
       if ((sweepers >> wave) & 1ull)
         wave_best([&](int k) { return min_keep(sqdist<D>(c[k], pa), dm[k]); }, lv, li, lc);
-      if (lane == 0) {
-        sh.la[par][0][wave] = lv;
-        sh.la[par][1][wave] = li;
-#pragma unroll
-        for (int j = 0; j < D; ++j) sh.la[par][2 + j][wave] = __float_as_int(lc[j]);
-      }
-      lds_barrier();
       int lw[NW];
+      if constexpr (VEC) {
+        if (lane == 0) {
+          sh.la4[par][wave][0] = make_int4(lv, li, __float_as_int(lc[0]), __float_as_int(lc[1]));
+          sh.la4[par][wave][1] = make_int4(__float_as_int(lc[2]), 0, 0, 0);
+        }
+        lds_barrier();
+        const int4 t0 = (lane < W) ? sh.la4[par][lane][0] : make_int4(INT_MIN, 0, 0, 0);
+        const int t1 = (lane < W) ? sh.la4[par][lane][1].x : 0;
+        lw[0] = t0.x;
+        lw[1] = t0.y;
+        lw[2] = t0.z;
+        lw[3] = t0.w;
+        lw[4] = t1;
+      } else {
+        if (lane == 0) {
+          sh.la[par][0][wave] = lv;
+          sh.la[par][1][wave] = li;
 #pragma unroll
-      for (int k = 0; k < NW; ++k) lw[k] = (lane < W) ? sh.la[par][k][lane] : 0;
-      if (lane >= W) lw[0] = INT_MIN;
+          for (int j = 0; j < D; ++j) sh.la[par][2 + j][wave] = __float_as_int(lc[j]);
+        }
+        lds_barrier();
+#pragma unroll
+        for (int k = 0; k < NW; ++k) lw[k] = (lane < W) ? sh.la[par][k][lane] : 0;
+        if (lane >= W) lw[0] = INT_MIN;
+      }
       int lgv;
       const int wl2 = wave_argmax_lane<TIE>(lw[0], (uint32_t)lw[1], lgv);
 #pragma unroll
@@ -431,20 +474,42 @@ __device__ __forceinline__ bool run_job_pp(const BatchPPParams &ap, uint32_t r,
         r2[j] = readlane_i(cw, l2);
       }
       if (lane == 0) {
+        if constexpr (VEC) {
+          sh.res4[par][0] = make_int4(r1[0], r1[1], r1[2], r1[3]);
+          sh.res4[par][1] = make_int4(r1[4], v2, (int)i2, r2[0]);
+          sh.res4[par][2] = make_int4(r2[1], r2[2], two ? 1 : 0, all_ok ? 1 : 0);
+        } else {
 #pragma unroll
-        for (int k = 0; k < NW; ++k) sh.res[par][k] = r1[k];
-        sh.res[par][NW] = v2;
-        sh.res[par][NW + 1] = (int)i2;
+          for (int k = 0; k < NW; ++k) sh.res[par][k] = r1[k];
+          sh.res[par][NW] = v2;
+          sh.res[par][NW + 1] = (int)i2;
 #pragma unroll
-        for (int j = 0; j < D; ++j) sh.res[par][NW + 2 + j] = r2[j];
-        sh.res[par][NP] = two ? 1 : 0;
-        sh.res[par][NP + 1] = all_ok ? 1 : 0;
+          for (int j = 0; j < D; ++j) sh.res[par][NW + 2 + j] = r2[j];
+          sh.res[par][NP] = two ? 1 : 0;
+          sh.res[par][NP + 1] = all_ok ? 1 : 0;
+        }
       }
     }
     lds_barrier();
     int res[NP + 2];
+    if constexpr (VEC) {
+      const int4 q0 = sh.res4[par][0], q1 = sh.res4[par][1], q2 = sh.res4[par][2];
+      res[0] = q0.x;
+      res[1] = q0.y;
+      res[2] = q0.z;
+      res[3] = q0.w;
+      res[4] = q1.x;
+      res[5] = q1.y;
+      res[6] = q1.z;
+      res[7] = q1.w;
+      res[8] = q2.x;
+      res[9] = q2.y;
+      res[10] = q2.z;
+      res[11] = q2.w;
+    } else {
 #pragma unroll
-    for (int k = 0; k < NP + 2; ++k) res[k] = sh.res[par][k];
+      for (int k = 0; k < NP + 2; ++k) res[k] = sh.res[par][k];
+    }
     if (res[NP + 1] == 0) return false;
     i1 = (uint32_t)__builtin_amdgcn_readfirstlane(res[1]);
 #pragma unroll
@@ -471,10 +536,7 @@ __device__ __forceinline__ bool run_job_pp(const BatchPPParams &ap, uint32_t r,
       const int ml = wave_argmax_lane<TIE>((keep && (uint32_t)lane != r) ? sv : INT_MIN,
                                            (uint32_t)si, mv);
       const int mi = readlane_i(si, ml);
-      if (lane == 0) {
-        sh.mu[par][0] = mv;
-        sh.mu[par][1] = mi;
-      }
+      if (lane == 0) sh.mu[par] = make_int2(mv, mi);
     }
     if (recorder && lane == 0) {
       a.out[J.out_off + s] = i1;
@@ -513,7 +575,7 @@ __device__ __forceinline__ bool run_job_pp(const BatchPPParams &ap, uint32_t r,
 // serving loop therefore runs the one-slot form.
 template <int B, int PPT, int TIE, bool ONE>
 __global__ void __launch_bounds__(B) fps_batch_pp(BatchPPParams ap) {
-  __shared__ PPShared<B, PPT, TIE> sh;
+  __shared__ PPShared<B, PPT, TIE, !ONE> sh;
   __shared__ uint32_t s_claim;
   __shared__ int s_ok;
   const BatchParams &a = ap.b;
@@ -555,7 +617,7 @@ __global__ void __launch_bounds__(B) fps_batch_pp(BatchPPParams ap) {
         return;
       }
     }
-    if (!run_job_pp<B, PPT, TIE>(ap, r, sh)) {
+    if (!run_job_pp<B, PPT, TIE, !ONE>(ap, r, sh)) {
       if (tid == 0) atomicOr(a.status, kStatusTimeout);
       return;
     }
