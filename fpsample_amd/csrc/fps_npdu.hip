@@ -227,10 +227,13 @@ __global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
   constexpr int W = B / 64;
   constexpr int NW = 2 + D;   // value, index, coordinates
   constexpr int NP = 2 * NW;  // words published per workgroup: A, then A'
-  __shared__ int s_top[2][NW + 1][W];  // wave bests + "changed" flags, by exchange parity
-  __shared__ int s_la[2][NW][W];       // wave lookahead bests
-  __shared__ int s_res[2][NP + 2];     // p1, p2, two-pick flag, all-arrived flag
-  __shared__ int s_mu[2][2];           // M of the round of this parity
+  // Hand-offs as 16-byte records, moved with 128-bit LDS accesses, by
+  // exchange parity: per wave its best and "changed" flag, its lookahead
+  // best; the round's result (p1, p2, two-pick flag, all-arrived flag); M.
+  __shared__ int4 s_top[2][W][2];
+  __shared__ int4 s_la[2][W][2];
+  __shared__ int4 s_res[2][4];
+  __shared__ int2 s_mu[2];
 
   const FpsParams &f = a.fps;
   const int tid = threadIdx.x;
@@ -357,10 +360,7 @@ __global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
   uint32_t w1l = 0, w1h = n - 1u, w2l = 0, w2h = 0;  // the start updates every point
   uint32_t e = 0;
   const int K = min(min((int)f.replicas, 64 / NP), 4);
-  if (tid < 2) {
-    s_mu[tid][0] = INT_MIN;
-    s_mu[tid][1] = 0;
-  }
+  if (tid < 2) s_mu[tid] = make_int2(INT_MIN, 0);
   lds_barrier();
 
   uint32_t s = 1;
@@ -375,19 +375,24 @@ __global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
     ++e;
     const int par = (int)(e & 1u);
     if (lane == 0) {
-      s_top[par][0][wave] = cv;
-      s_top[par][1][wave] = ci;
-#pragma unroll
-      for (int j = 0; j < D; ++j) s_top[par][2 + j][wave] = __float_as_int(cc[j]);
-      s_top[par][NW][wave] = upd ? 1 : 0;
+      s_top[par][wave][0] = make_int4(cv, ci, __float_as_int(cc[0]), __float_as_int(cc[1]));
+      s_top[par][wave][1] = make_int4(__float_as_int(cc[2]), upd ? 1 : 0, 0, 0);
     }
     lds_barrier();
     int tw[NW + 1];
-#pragma unroll
-    for (int k = 0; k <= NW; ++k) tw[k] = (lane < W) ? s_top[par][k][lane] : 0;
-    if (lane >= W) tw[0] = INT_MIN;
-    mu_v = s_mu[par ^ 1][0];
-    mu_i = (uint32_t)s_mu[par ^ 1][1];
+    {
+      const int4 t0 = (lane < W) ? s_top[par][lane][0] : make_int4(INT_MIN, 0, 0, 0);
+      const int4 t1 = (lane < W) ? s_top[par][lane][1] : make_int4(0, 0, 0, 0);
+      tw[0] = t0.x;
+      tw[1] = t0.y;
+      tw[2] = t0.z;
+      tw[3] = t0.w;
+      tw[4] = t1.x;
+      tw[5] = t1.y;
+      const int2 mu = s_mu[par ^ 1];
+      mu_v = mu.x;
+      mu_i = (uint32_t)mu.y;
+    }
     const bool changed = e == 1u || __ballot(lane < W && tw[NW] != 0) != 0;
     if (changed) {
       int gv;
@@ -433,16 +438,20 @@ __global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
             lv, li, lc);
       }
       if (lane == 0) {
-        s_la[par][0][wave] = lv;
-        s_la[par][1][wave] = li;
-#pragma unroll
-        for (int j = 0; j < D; ++j) s_la[par][2 + j][wave] = __float_as_int(lc[j]);
+        s_la[par][wave][0] = make_int4(lv, li, __float_as_int(lc[0]), __float_as_int(lc[1]));
+        s_la[par][wave][1] = make_int4(__float_as_int(lc[2]), 0, 0, 0);
       }
       lds_barrier();
       int lw[NW];
-#pragma unroll
-      for (int k = 0; k < NW; ++k) lw[k] = (lane < W) ? s_la[par][k][lane] : 0;
-      if (lane >= W) lw[0] = INT_MIN;
+      {
+        const int4 t0 = (lane < W) ? s_la[par][lane][0] : make_int4(INT_MIN, 0, 0, 0);
+        const int t1 = (lane < W) ? s_la[par][lane][1].x : 0;
+        lw[0] = t0.x;
+        lw[1] = t0.y;
+        lw[2] = t0.z;
+        lw[3] = t0.w;
+        lw[4] = t1;
+      }
       int lgv;
       const int wl2 = wave_argmax_lane<TIE>(lw[0], (uint32_t)lw[1], lgv);
 #pragma unroll
@@ -537,20 +546,28 @@ __global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
         r2[j] = readlane_i(cw, l2);
       }
       if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NW; ++k) s_res[par][k] = r1[k];
-        s_res[par][NW] = v2;
-        s_res[par][NW + 1] = (int)i2;
-#pragma unroll
-        for (int j = 0; j < D; ++j) s_res[par][NW + 2 + j] = r2[j];
-        s_res[par][NP] = two ? 1 : 0;
-        s_res[par][NP + 1] = all_ok ? 1 : 0;
+        s_res[par][0] = make_int4(r1[0], r1[1], r1[2], r1[3]);
+        s_res[par][1] = make_int4(r1[4], v2, (int)i2, r2[0]);
+        s_res[par][2] = make_int4(r2[1], r2[2], two ? 1 : 0, all_ok ? 1 : 0);
       }
     }
     lds_barrier();
     int res[NP + 2];
-#pragma unroll
-    for (int k = 0; k < NP + 2; ++k) res[k] = s_res[par][k];
+    {
+      const int4 q0 = s_res[par][0], q1 = s_res[par][1], q2 = s_res[par][2];
+      res[0] = q0.x;
+      res[1] = q0.y;
+      res[2] = q0.z;
+      res[3] = q0.w;
+      res[4] = q1.x;
+      res[5] = q1.y;
+      res[6] = q1.z;
+      res[7] = q1.w;
+      res[8] = q2.x;
+      res[9] = q2.y;
+      res[10] = q2.z;
+      res[11] = q2.w;
+    }
     if (res[NP + 1] == 0) {
       if (tid == 0) atomicOr(f.status, kStatusTimeout);
       return;
@@ -576,10 +593,7 @@ __global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
       int mv;
       const int ml = wave_argmax_lane<TIE>(keep ? sv : INT_MIN, (uint32_t)si, mv);
       const int mi = readlane_i(si, ml);
-      if (lane == 0) {
-        s_mu[par][0] = mv;
-        s_mu[par][1] = mi;
-      }
+      if (lane == 0) s_mu[par] = make_int2(mv, mi);
     }
     if (recorder) {
       record_pick<D>(f, s, i1, p1);
