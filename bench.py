@@ -385,7 +385,7 @@ def roofline(workload: str, plan: dict, n: int, m: int, kernel_s: float, npdu: b
         words = 5
     else:
         exchanges = launch_exchanges()
-        words = 10 if kind == "pair" else 5
+        words = 10 if kind in ("pair", "spair") else 5
     floor_us = probe_exchange(grid, words)
     if exchanges is None:
         out.update(bound="latency", achieved=None, peak=round(1e6 / floor_us, 1),

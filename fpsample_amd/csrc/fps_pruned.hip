@@ -1276,6 +1276,9 @@ __global__ void __launch_bounds__(1024) fps_stream_pruned(PrunedParams a) {
   uint32_t e = 0;
   for (uint32_t s = 1; s < f.m; ++s) {
     const int sp = (int)(s & 1u);
+    // This wave's min-distance stores of the previous pass complete before
+    // the barrier below, after which any wave may read those blocks.
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     // Box tests: wave w takes slots [64 w, 64 w + 64), [64 w + 64 W, ...).
     for (uint32_t j0 = (uint32_t)(wave * 64); j0 < nbl; j0 += 64u * W) {
       const uint32_t jl = j0 + (uint32_t)lane;
@@ -1386,27 +1389,44 @@ __global__ void __launch_bounds__(1024) fps_stream_pruned(PrunedParams a) {
     }
     ++e;
     const int par = (int)(e & 1u);
-    // Workgroup argmax over the cached candidates (every wave).
-    int bv = INT_MIN;
-    uint32_t bi = 0;
-    uint32_t bj = 0;
-    for (uint32_t j = (uint32_t)lane; j < nbl; j += 64u) {
-      const int v = m_v[j];
-      const uint32_t id = (uint32_t)m_i[j];
-      if (beats<TIE>(v, id, bv, bi)) {
-        bv = v;
-        bi = id;
-        bj = j;
-      }
-    }
-    int gv;
-    const int gl = wave_argmax_lane<TIE>(bv, bi, gv);
-    const uint32_t wj = (uint32_t)readlane_i((int)bj, gl);
+    // Workgroup argmax over the cached candidates, eight slots per lane
+    // loaded at once: by wave 0 alone, which publishes it, unless there is no
+    // exchange (one workgroup: every wave).
     int win[NW];
-    win[0] = gv;
-    win[1] = readlane_i((int)bi, gl);
 #pragma unroll
-    for (int q = 0; q < D; ++q) win[2 + q] = m_c[q * nbl + wj];
+    for (int q = 0; q < NW; ++q) win[q] = 0;
+    if (G == 1 || wave == 0) {
+      int bv = INT_MIN;
+      uint32_t bi = 0;
+      uint32_t bj = 0;
+      for (uint32_t j0 = 0; j0 < nbl; j0 += 64u * 8u) {
+        int v[8];
+        uint32_t id[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const uint32_t jl = j0 + (uint32_t)(u * 64 + lane);
+          const uint32_t jc = jl < nbl ? jl : nbl - 1u;
+          v[u] = m_v[jc];
+          id[u] = (uint32_t)m_i[jc];
+          v[u] = jl < nbl ? v[u] : INT_MIN;  // padding never beats a real slot
+          id[u] = jl < nbl ? id[u] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const bool take = beats<TIE>(v[u], id[u], bv, bi);
+          bv = take ? v[u] : bv;
+          bi = take ? id[u] : bi;
+          bj = take ? j0 + (uint32_t)(u * 64 + lane) : bj;
+        }
+      }
+      int gv;
+      const int gl = wave_argmax_lane<TIE>(bv, bi, gv);
+      const uint32_t wj = (uint32_t)readlane_i((int)bj, gl);
+      win[0] = gv;
+      win[1] = readlane_i((int)bi, gl);
+#pragma unroll
+      for (int q = 0; q < D; ++q) win[2 + q] = m_c[q * nbl + wj];
+    }
     if (G == 1) {
       lds_barrier();  // every wave is done reading the cache before the next update
     } else {
@@ -1431,6 +1451,484 @@ __global__ void __launch_bounds__(1024) fps_stream_pruned(PrunedParams a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Paired streaming form: the blocks and per-block caches of fps_stream_pruned,
+// settling up to two picks per exchange as fps_pruned_pair does.  Every
+// workgroup publishes its best A and a lookahead A' (its best after a
+// hypothetical update by A); the polled pairs give p1 and, unless some
+// workgroup whose A moves under p1 could still beat it, p2.  For the
+// lookahead every wave tests its share of the blocks: a block whose box lies
+// out of A's reach, or whose cached best keeps its value under A, keeps its
+// best; the others are listed and streamed, one per wave (their min-distances
+// min'ed again with this pass's pivots -- a no-op once this pass's stores
+// have landed -- and with A; nothing written back), and wave 0 takes the best
+// of the waves' results.  A workgroup skips the lookahead while
+// the best A of the workgroups the last picks certainly left unchanged (their
+// A keeps its value, so it stays their best) beats its own: it cannot then
+// hold the next p1, the only workgroup whose A' is read.  2 <= G <= 64.
+// ---------------------------------------------------------------------------
+template <int TIE>
+__global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
+  constexpr int B = 1024;
+  constexpr int W = B / 64;
+  constexpr int D = 3;
+  constexpr int NW = 2 + D;
+  constexpr int NP = 2 * NW;
+  constexpr int SL = kSBlock / 64;  // slots per lane in a block
+  __shared__ int4 s_res[2][3];      // p1, p2, two-pick flag, all-arrived flag
+  extern __shared__ int s_meta[];   // kSMetaWords rows of nbl words, then the block list
+  __shared__ int s_count[2];        // listed blocks, by pass parity
+  __shared__ int4 s_a[2];           // A (value, index, coordinates), lookahead flag
+  __shared__ int s_cnt2;            // blocks the lookahead streams
+  __shared__ int4 s_lp[W][2];       // per wave: best kept candidate (L)
+  __shared__ int4 s_hp[W][2];       // per wave: best of its streamed blocks
+
+  const FpsParams &f = a.fps;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const uint32_t G = gridDim.x;
+  const uint32_t g = blockIdx.x;
+  const uint32_t n = f.n;
+  const uint32_t nblocks = (n + kSBlock - 1) / kSBlock;
+  const uint32_t nbl = (nblocks > g) ? (nblocks - g + G - 1) / G : 0u;
+  float *m_lo = reinterpret_cast<float *>(s_meta);  // m_lo[c * nbl + j]
+  float *m_hi = m_lo + 3 * nbl;
+  int *m_v = reinterpret_cast<int *>(m_hi + 3 * nbl);
+  int *m_i = m_v + nbl;
+  int *m_c = m_i + nbl;  // m_c[c * nbl + j]: coordinate bits of the block's best
+  int *s_list = m_c + 3 * nbl;  // blocks to stream (update, then wave 0's lookahead)
+  if (tid < 2) s_count[tid] = 0;
+  lds_barrier();
+  const float *xs = a.sorted;
+  const float *ys = a.sorted + a.sorted_ld;
+  const float *zs = a.sorted + 2 * a.sorted_ld;
+  const bool recorder = (g == 0 && tid >= B - 64);
+
+  float p1[D], p2[D];
+  uint32_t i1 = load_forced<D>(f, 0, p1);
+#pragma unroll
+  for (int j = 0; j < D; ++j) p2[j] = 0.0f;
+  if (recorder) record_pick<D>(f, 0, i1, p1);
+
+  // Wave 0's state: the workgroup's A and A', M (best A among the other
+  // workgroups the last picks left unchanged).
+  int ga[NW], gl_[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    ga[k] = 0;
+    gl_[k] = 0;
+  }
+  bool la_valid = false;
+  int mu_v = INT_MIN;
+  uint32_t mu_i = 0;
+  const int K = min(min((int)f.replicas, 64 / NP), 4);
+
+  bool have2 = false;
+  bool first = true;
+  uint32_t e = 0;
+  uint32_t t = 0;
+  uint32_t s = 1;
+  while (s < f.m) {
+    const int sp = (int)(t & 1u);
+    ++t;
+    // This wave's min-distance stores of the previous pass complete before
+    // the barrier below, after which any wave may read those blocks.
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // Box tests against the pending pivots.
+    for (uint32_t j0 = (uint32_t)(wave * 64); j0 < nbl; j0 += 64u * W) {
+      const uint32_t jl = j0 + (uint32_t)lane;
+      bool need = false;
+      if (jl < nbl) {
+        if (first) {
+          need = true;
+        } else {
+          const float lo[D] = {m_lo[jl], m_lo[nbl + jl], m_lo[2 * nbl + jl]};
+          const float hi[D] = {m_hi[jl], m_hi[nbl + jl], m_hi[2 * nbl + jl]};
+          const float bvf = __int_as_float(m_v[jl]);
+          need = box_bound<D>(p1, lo, hi) < bvf || (have2 && box_bound<D>(p2, lo, hi) < bvf);
+        }
+      }
+      const uint64_t mask = __ballot(need);
+      if (mask != 0) {
+        int at = 0;
+        if (lane == 0) at = atomicAdd(&s_count[sp], __popcll(mask));
+        at = readlane_i(at, 0);
+        if (need) s_list[at + __popcll(mask & ((1ull << lane) - 1))] = (int)jl;
+      }
+    }
+    lds_barrier();  // the list is complete
+    if (tid == 0) s_count[sp ^ 1] = 0;  // next pass's counter; nobody reads it before then
+    const int listed = s_count[sp];
+    // Update: the waves take the listed blocks in turn (both pivots).
+    for (int u = wave; u < listed; u += W) {
+      const uint32_t j = (uint32_t)s_list[u];
+      const uint32_t base = (g + j * G) * (uint32_t)kSBlock;
+      int bv = INT_MIN;
+      uint32_t bi = 0;
+      float bc[D] = {0.0f, 0.0f, 0.0f};
+      float lo[D] = {INFINITY, INFINITY, INFINITY};
+      float hi[D] = {-INFINITY, -INFINITY, -INFINITY};
+      float bx[SL][D], bo[SL];
+      uint32_t bid[SL];
+#pragma unroll
+      for (int k = 0; k < SL; ++k) {
+        const uint32_t p = base + (uint32_t)(k * 64 + lane);
+        const uint32_t q = p < n ? p : n - 1;
+        bx[k][0] = xs[q];
+        bx[k][1] = ys[q];
+        bx[k][2] = zs[q];
+        bid[k] = a.perm[q];
+        bo[k] = first ? f.init : f.dmin[q];
+      }
+#pragma unroll
+      for (int k = 0; k < SL; ++k) {
+        const uint32_t p = base + (uint32_t)(k * 64 + lane);
+        const bool live = p < n;
+        const float(&c)[D] = bx[k];
+        const float old = bo[k];
+        float nm = min_keep(sqdist<D>(c, p1), old);
+        if (have2) nm = min_keep(sqdist<D>(c, p2), nm);
+        if (live && (first || __float_as_int(nm) != __float_as_int(old))) f.dmin[p] = nm;
+        const int b = __float_as_int(nm);
+        const bool take = live && ((TIE == kTieHighest) ? (b >= bv) : (b > bv));
+        bv = take ? b : bv;
+        bi = take ? bid[k] : bi;
+#pragma unroll
+        for (int r = 0; r < D; ++r) bc[r] = take ? c[r] : bc[r];
+        if (first && live) {
+#pragma unroll
+          for (int r = 0; r < D; ++r) {
+            lo[r] = fminf(lo[r], c[r]);
+            hi[r] = fmaxf(hi[r], c[r]);
+          }
+        }
+      }
+      int wv;
+      const int wl = wave_argmax_lane<TIE>(bv, bi, wv);
+      const int wi = readlane_i((int)bi, wl);
+      float wc[D];
+#pragma unroll
+      for (int q = 0; q < D; ++q) wc[q] = readlane_f(bc[q], wl);
+      if (first) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+          lo[q] = wave_min_f(lo[q]);
+          hi[q] = wave_max_f(hi[q]);
+        }
+      }
+      if (lane == 0) {
+        m_v[j] = wv;
+        m_i[j] = wi;
+#pragma unroll
+        for (int q = 0; q < D; ++q) m_c[q * nbl + j] = __float_as_int(wc[q]);
+        if (first) {
+#pragma unroll
+          for (int q = 0; q < D; ++q) {
+            m_lo[q * nbl + j] = lo[q];
+            m_hi[q * nbl + j] = hi[q];
+          }
+        }
+      }
+    }
+    lds_barrier();  // every block's cached candidate is current
+    if (s < f.n_starts) {
+      // Forced prefix (multi-start): one pick per pass, no exchange.
+      i1 = load_forced<D>(f, s, p1);
+      have2 = false;
+      if (recorder) record_pick<D>(f, s, i1, p1);
+      ++s;
+      first = false;
+      continue;
+    }
+    ++e;
+    const int par = (int)(e & 1u);
+    if (wave == 0) {
+      if (e == 1u || listed > 0) {
+        // The workgroup's A: argmax over the cached candidates, eight slots
+        // per lane loaded at once.
+        int bv = INT_MIN;
+        uint32_t bi = 0, bj = 0;
+        for (uint32_t j0 = 0; j0 < nbl; j0 += 64u * 8u) {
+          int v[8];
+          uint32_t id[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const uint32_t jl = j0 + (uint32_t)(u * 64 + lane);
+            const uint32_t jc = jl < nbl ? jl : nbl - 1u;
+            v[u] = m_v[jc];
+            id[u] = (uint32_t)m_i[jc];
+            v[u] = jl < nbl ? v[u] : INT_MIN;  // padding never beats a real slot
+            id[u] = jl < nbl ? id[u] : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const bool take = beats<TIE>(v[u], id[u], bv, bi);
+            bv = take ? v[u] : bv;
+            bi = take ? id[u] : bi;
+            bj = take ? j0 + (uint32_t)(u * 64 + lane) : bj;
+          }
+        }
+        int gv;
+        const int gl = wave_argmax_lane<TIE>(bv, bi, gv);
+        const uint32_t wj = (uint32_t)readlane_i((int)bj, gl);
+        ga[0] = gv;
+        ga[1] = readlane_i((int)bi, gl);
+#pragma unroll
+        for (int q = 0; q < D; ++q) ga[2 + q] = m_c[q * nbl + wj];
+        la_valid = false;
+      }
+      // Lookahead only while this workgroup could hold the next p1.
+      const bool needL = !la_valid && beats<TIE>(ga[0], (uint32_t)ga[1], mu_v, mu_i);
+      if (needL) la_valid = true;
+      if (lane == 0) {
+        s_a[0] = make_int4(ga[0], ga[1], ga[2], ga[3]);
+        s_a[1] = make_int4(ga[4], needL ? 1 : 0, 0, 0);
+        s_cnt2 = 0;
+      }
+    }
+    lds_barrier();  // A and the lookahead decision
+    if (s_a[1].y != 0) {
+      // Lookahead, every wave.  L: the best cached candidate of the blocks A
+      // leaves unchanged at their best (out of reach, or their best keeps its
+      // value); the other blocks are listed and streamed, one per wave.
+      const int4 sa0 = s_a[0], sa1 = s_a[1];
+      const float pa[D] = {__int_as_float(__builtin_amdgcn_readfirstlane(sa0.z)),
+                           __int_as_float(__builtin_amdgcn_readfirstlane(sa0.w)),
+                           __int_as_float(__builtin_amdgcn_readfirstlane(sa1.x))};
+      int lv = INT_MIN;
+      uint32_t li = 0;
+      int lc[D] = {0, 0, 0};
+      for (uint32_t j0 = (uint32_t)(wave * 64); j0 < nbl; j0 += 64u * W) {
+        const uint32_t jl = j0 + (uint32_t)lane;
+        bool sweep = false;
+        if (jl < nbl) {
+          const int v = m_v[jl];
+          const uint32_t id = (uint32_t)m_i[jl];
+          const float lo[D] = {m_lo[jl], m_lo[nbl + jl], m_lo[2 * nbl + jl]};
+          const float hi[D] = {m_hi[jl], m_hi[nbl + jl], m_hi[2 * nbl + jl]};
+          const int cb[D] = {m_c[jl], m_c[nbl + jl], m_c[2 * nbl + jl]};
+          const float cf[D] = {__int_as_float(cb[0]), __int_as_float(cb[1]), __int_as_float(cb[2])};
+          const float vf = __int_as_float(v);
+          const bool kept = !(box_bound<D>(pa, lo, hi) < vf) ||
+                            __float_as_int(min_keep(sqdist<D>(cf, pa), vf)) == v;
+          if (kept && beats<TIE>(v, id, lv, li)) {
+            lv = v;
+            li = id;
+#pragma unroll
+            for (int q = 0; q < D; ++q) lc[q] = cb[q];
+          }
+          sweep = !kept;
+        }
+        const uint64_t mask = __ballot(sweep);
+        if (mask != 0) {
+          int at = 0;
+          if (lane == 0) at = atomicAdd(&s_cnt2, __popcll(mask));
+          at = readlane_i(at, 0);
+          if (sweep) s_list[at + __popcll(mask & ((1ull << lane) - 1))] = (int)jl;
+        }
+      }
+      {
+        int Lv;
+        const int ll = wave_argmax_lane<TIE>(lv, li, Lv);
+        const int Li = readlane_i((int)li, ll);
+        int Lc[D];
+#pragma unroll
+        for (int q = 0; q < D; ++q) Lc[q] = readlane_i(lc[q], ll);
+        if (lane == 0) {
+          s_lp[wave][0] = make_int4(Lv, Li, Lc[0], Lc[1]);
+          s_lp[wave][1] = make_int4(Lc[2], 0, 0, 0);
+        }
+      }
+      lds_barrier();  // the list and every wave's L
+      const int n2 = s_cnt2;
+      int hv = INT_MIN;  // this lane's best of the streamed blocks
+      uint32_t hid = 0;
+      float hc[D] = {0.0f, 0.0f, 0.0f};
+      for (int u = wave; u < n2; u += W) {
+        const uint32_t j = (uint32_t)s_list[u];
+        const uint32_t base = (g + j * G) * (uint32_t)kSBlock;
+        float bx[SL][D], bo[SL];
+        uint32_t bid[SL];
+#pragma unroll
+        for (int k = 0; k < SL; ++k) {
+          const uint32_t p = base + (uint32_t)(k * 64 + lane);
+          const uint32_t q = p < n ? p : n - 1;
+          bx[k][0] = xs[q];
+          bx[k][1] = ys[q];
+          bx[k][2] = zs[q];
+          bid[k] = a.perm[q];
+          // First pass: the stores of this pass may not have landed, and
+          // what was there before is not a min-distance.
+          bo[k] = first ? f.init : f.dmin[q];
+        }
+#pragma unroll
+        for (int k = 0; k < SL; ++k) {
+          const uint32_t p = base + (uint32_t)(k * 64 + lane);
+          float h = min_keep(sqdist<D>(bx[k], p1), bo[k]);
+          if (have2) h = min_keep(sqdist<D>(bx[k], p2), h);
+          h = min_keep(sqdist<D>(bx[k], pa), h);
+          const int b = __float_as_int(h);
+          const bool take = p < n && beats<TIE>(b, bid[k], hv, hid);
+          hv = take ? b : hv;
+          hid = take ? bid[k] : hid;
+#pragma unroll
+          for (int r = 0; r < D; ++r) hc[r] = take ? bx[k][r] : hc[r];
+        }
+      }
+      {
+        int Hv;
+        const int hl = wave_argmax_lane<TIE>(hv, hid, Hv);
+        const int Hi = readlane_i((int)hid, hl);
+        int Hc[D];
+#pragma unroll
+        for (int q = 0; q < D; ++q) Hc[q] = __float_as_int(readlane_f(hc[q], hl));
+        if (lane == 0) {
+          s_hp[wave][0] = make_int4(Hv, Hi, Hc[0], Hc[1]);
+          s_hp[wave][1] = make_int4(Hc[2], 0, 0, 0);
+        }
+      }
+      lds_barrier();  // every wave's streamed best
+      if (wave == 0) {
+        // A' = the best of the waves' L's and streamed bests, one per lane.
+        const bool has = lane < 2 * W;
+        const int r = has ? (lane < W ? lane : lane - W) : 0;
+        const int4 c0 = lane < W ? s_lp[r][0] : s_hp[r][0];
+        const int c4 = lane < W ? s_lp[r][1].x : s_hp[r][1].x;
+        int av;
+        const int al = wave_argmax_lane<TIE>(has ? c0.x : INT_MIN, (uint32_t)c0.y, av);
+        gl_[0] = av;
+        gl_[1] = readlane_i(c0.y, al);
+        gl_[2] = readlane_i(c0.z, al);
+        gl_[3] = readlane_i(c0.w, al);
+        gl_[4] = readlane_i(c4, al);
+      }
+    }
+    if (wave == 0) {
+      // Exchange: publish the pair to K replicas, poll replica g mod K, one
+      // workgroup per lane, all NP words at once.
+      uint64_t *bufs = f.xchg + (size_t)(e & 1u) * f.replicas * kSlotWords * kPageWords;
+      if (lane < K * NP) {
+        const int rep = lane / NP;
+        const int k = lane - rep * NP;
+        int word = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          int wa = ga[q], wb = gl_[q];
+          asm volatile("" : "+v"(wa), "+v"(wb));
+          word = (k == q) ? wa : word;
+          word = (k == NW + q) ? wb : word;
+        }
+        uint32_t poff = (uint32_t)(((rep * kSlotWords + k) * kPageWords + (int)g) * sizeof(uint64_t));
+        asm volatile("" : "+v"(poff));
+        xchg_store(reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(bufs) + poff),
+                   ((uint64_t)(uint32_t)word << 32) | e);
+      }
+      const uint64_t *buf = bufs + (size_t)(g % (uint32_t)K) * kSlotWords * kPageWords;
+      uint32_t loff = (uint32_t)lane * (uint32_t)sizeof(uint64_t);
+      int x[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) x[k] = 0;
+      bool ok = true;
+      const bool live = (uint32_t)lane < G;
+      if (live) {
+        uint64_t w[NP];
+        const uint64_t deadline = now_ticks() + f.spin_ticks;
+        for (;;) {
+          bool all = true;
+          asm volatile("" : "+v"(loff));
+#pragma unroll
+          for (int k = 0; k < NP; ++k)
+            w[k] = xchg_load(reinterpret_cast<const uint64_t *>(
+                reinterpret_cast<const char *>(buf + (size_t)k * kPageWords) + loff));
+#pragma unroll
+          for (int k = 0; k < NP; ++k) all = all && ((uint32_t)w[k] == e);
+          if (all) break;
+          if (now_ticks() > deadline) {
+            ok = false;
+            break;
+          }
+          for (uint32_t z = 0; z < f.backoff; ++z) __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) x[k] = (int)(w[k] >> 32);
+      }
+      const bool valid = live && ok;
+      // p1: the best A.
+      int v1;
+      const int l1 = wave_argmax_lane<TIE>(valid ? x[0] : INT_MIN, (uint32_t)x[1], v1);
+      int r1[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) r1[k] = readlane_i(x[k], l1);
+      float q1[D], qa[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        q1[j] = __int_as_float(r1[2 + j]);
+        qa[j] = __int_as_float(x[2 + j]);
+      }
+      // Each A after p1, in the update's own arithmetic.
+      const float post = min_keep(sqdist<D>(qa, q1), __int_as_float(x[0]));
+      const bool moved = valid && lane != l1 && __float_as_int(post) != x[0];
+      const bool mine = lane == l1;
+      const int tv = mine ? x[NW] : ((valid && !moved) ? x[0] : INT_MIN);
+      const uint32_t ti = (uint32_t)(mine ? x[NW + 1] : x[1]);
+      int v2;
+      const int l2 = wave_argmax_lane<TIE>(tv, ti, v2);
+      const uint32_t i2 = (uint32_t)readlane_i((int)ti, l2);
+      const bool beaten = moved && !beats<TIE>(v2, i2, x[0], (uint32_t)x[1]);
+      const bool two = __ballot(beaten) == 0;
+      const bool all_ok = __ballot(live && !ok) == 0;
+      int r2[D];
+      float q2[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const int cw = mine ? x[NW + 2 + j] : x[2 + j];
+        r2[j] = readlane_i(cw, l2);
+        q2[j] = __int_as_float(r2[j]);
+      }
+      if (lane == 0) {
+        s_res[par][0] = make_int4(r1[0], r1[1], r1[2], r1[3]);
+        s_res[par][1] = make_int4(r1[4], v2, (int)i2, r2[0]);
+        s_res[par][2] = make_int4(r2[1], r2[2], two ? 1 : 0, all_ok ? 1 : 0);
+      }
+      // M for the next pass: the best A among the other workgroups whose A
+      // keeps its value under this pass's picks (so stays their best).
+      const bool keep1 = __float_as_int(post) == x[0];
+      const float post2 = min_keep(sqdist<D>(qa, q2), __int_as_float(x[0]));
+      const bool keep = valid && x[0] != INT_MIN && keep1 &&
+                        !(two && __float_as_int(post2) != x[0]);
+      int mv;
+      const int ml = wave_argmax_lane<TIE>((keep && (uint32_t)lane != g) ? x[0] : INT_MIN,
+                                           (uint32_t)x[1], mv);
+      mu_v = mv;
+      mu_i = (uint32_t)readlane_i(x[1], ml);
+    }
+    lds_barrier();
+    const int4 q0 = s_res[par][0], qq1 = s_res[par][1], qq2 = s_res[par][2];
+    if (qq2.w == 0) {
+      if (tid == 0) atomicOr(f.status, kStatusTimeout);
+      return;
+    }
+    i1 = (uint32_t)__builtin_amdgcn_readfirstlane(q0.y);
+    p1[0] = __int_as_float(__builtin_amdgcn_readfirstlane(q0.z));
+    p1[1] = __int_as_float(__builtin_amdgcn_readfirstlane(q0.w));
+    p1[2] = __int_as_float(__builtin_amdgcn_readfirstlane(qq1.x));
+    const uint32_t i2 = (uint32_t)__builtin_amdgcn_readfirstlane(qq1.z);
+    p2[0] = __int_as_float(__builtin_amdgcn_readfirstlane(qq1.w));
+    p2[1] = __int_as_float(__builtin_amdgcn_readfirstlane(qq2.x));
+    p2[2] = __int_as_float(__builtin_amdgcn_readfirstlane(qq2.y));
+    have2 = qq2.z != 0 && s + 1 < f.m;
+    if (recorder) {
+      record_pick<D>(f, s, i1, p1);
+      if (have2) record_pick<D>(f, s + 1, i2, p2);
+    }
+    s += have2 ? 2u : 1u;
+    first = false;
+  }
+  // The launch's exchange count in the second status word.
+  if (recorder && lane == 0) f.status[1] = e;
+}
+
 static const KernelEntry kStreamPruned[] = {
     {3, 1024, 0, kTieHighest, (const void *)&fps_stream_pruned<kTieHighest>},
     {3, 1024, 0, kTieLowest, (const void *)&fps_stream_pruned<kTieLowest>},
@@ -1439,6 +1937,16 @@ static const KernelEntry kStreamPruned[] = {
 const KernelEntry *stream_pruned_table(size_t *count) {
   *count = sizeof(kStreamPruned) / sizeof(kStreamPruned[0]);
   return kStreamPruned;
+}
+
+static const KernelEntry kStreamPair[] = {
+    {3, 1024, 0, kTieHighest, (const void *)&fps_stream_pair<kTieHighest>},
+    {3, 1024, 0, kTieLowest, (const void *)&fps_stream_pair<kTieLowest>},
+};
+
+const KernelEntry *stream_pair_table(size_t *count) {
+  *count = sizeof(kStreamPair) / sizeof(kStreamPair[0]);
+  return kStreamPair;
 }
 
 #define FPSX_PRUNED(D, B, P)                                                   \

@@ -193,10 +193,12 @@ static int plan_stream(Ctx &c, size_t n, size_t dim, int tie, int B, size_t want
 // Streaming pruned plan: 1024-thread workgroups; G = 64 (lattice16m at 32,
 // 64, 128 and 256 workgroups: 23.2, 20.3, 22.2 and 26.2 ms), more where a
 // workgroup would cache over 2048 blocks (96 KB of LDS metadata), bounded by
-// co-residency and the exchange limits.  want_g > 0 forces G.
-static int plan_stream_pruned(Ctx &c, size_t n, int tie, size_t want_g, Plan &p) {
+// co-residency and the exchange limits.  want_g > 0 forces G.  pair: the
+// paired form, which needs 2 <= G <= 64.
+static int plan_stream_pruned(Ctx &c, size_t n, int tie, size_t want_g, Plan &p,
+                              bool pair = false) {
   size_t cnt;
-  const KernelEntry *t = stream_pruned_table(&cnt);
+  const KernelEntry *t = pair ? stream_pair_table(&cnt) : stream_pruned_table(&cnt);
   const KernelEntry *k = nullptr;
   for (size_t i = 0; i < cnt; ++i)
     if (t[i].tie == tie) k = &t[i];
@@ -205,12 +207,14 @@ static int plan_stream_pruned(Ctx &c, size_t n, int tie, size_t want_g, Plan &p)
   size_t g = want_g ? want_g
                     : std::max<size_t>(std::min<size_t>(64, nblocks), (nblocks + 2047) / 2048);
   g = std::min<size_t>(g, std::min<size_t>(nblocks, 256));
+  if (pair && (g < 2 || g > kPairMaxGrid))
+    return fail(FPSX_ETOOBIG, "paired streaming plan: needs 2..64 workgroups");
   size_t nbl = (nblocks + g - 1) / g;
   size_t shmem = (size_t)kSMetaWords * nbl * sizeof(int);
   if (shmem > 96 * 1024) return fail(FPSX_ETOOBIG, "streaming pruned plan: too many blocks per workgroup");
   const size_t gmax = (size_t)c.max_blocks(k, shmem);
   if (g > gmax) return fail(FPSX_ETOOBIG, "streaming pruned plan: grid exceeds co-residency");
-  p.kind = PlanKind::StreamPruned;
+  p.kind = pair ? PlanKind::StreamPair : PlanKind::StreamPruned;
   p.k = k;
   p.block = 1024;
   p.grid = (uint32_t)g;
@@ -248,6 +252,11 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
       if (dim != 3) return fail(FPSX_EINVAL, "FPSX_PLAN: the streaming pruned kernel is 3-D only");
       int got = std::sscanf(s.c_str(), "spruned:%d", &g);
       return plan_stream_pruned(c, n, tie, got == 1 ? (size_t)g : 0, p);
+    }
+    if (s.rfind("spair", 0) == 0) {
+      if (dim != 3) return fail(FPSX_EINVAL, "FPSX_PLAN: the streaming pruned kernel is 3-D only");
+      int got = std::sscanf(s.c_str(), "spair:%d", &g);
+      return plan_stream_pruned(c, n, tie, got == 1 ? (size_t)g : 0, p, true);
     }
     int pw = 0;
     if (std::sscanf(s.c_str(), "pairw%d:%d:%d", &pw, &a, &b) == 3) {
@@ -358,8 +367,12 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
     return FPSX_OK;
   // Beyond the register file: 3-D clouds stream only the blocks a pick can
   // change; other dimensions stream everything.
-  if (dim == 3 && m >= 64 && !(np && np[0] == '1') && plan_stream_pruned(c, n, tie, 0, p) == FPSX_OK)
-    return FPSX_OK;
+  // The paired form first (up to 64 workgroups, i.e. 64 M points).
+  if (dim == 3 && m >= 64 && !(np && np[0] == '1')) {
+    if (!(npair && npair[0] == '1') && plan_stream_pruned(c, n, tie, 0, p, true) == FPSX_OK)
+      return FPSX_OK;
+    if (plan_stream_pruned(c, n, tie, 0, p) == FPSX_OK) return FPSX_OK;
+  }
   return plan_stream(c, n, dim, tie, 1024, 0, false, p);
 }
 
@@ -550,6 +563,7 @@ static std::vector<Plan> candidates(Ctx &c, size_t n, size_t dim, int tie, size_
         out.push_back(q);
     }
     if (plan_stream_pruned(c, n, tie, 0, q) == FPSX_OK) out.push_back(q);
+    if (plan_stream_pruned(c, n, tie, 0, q, true) == FPSX_OK) out.push_back(q);
   }
   for (int B : {256, 1024}) {
     if (plan_stream(c, n, dim, tie, B, 0, false, q) == FPSX_OK) out.push_back(q);
@@ -658,16 +672,16 @@ int launch_fps(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld, 
   }
   PrunedParams pa;
   void *arg = &a;
-  if (p.kind == PlanKind::Pruned || p.kind == PlanKind::Pair || p.kind == PlanKind::StreamPruned) {
+  const bool streamed = p.kind == PlanKind::StreamPruned || p.kind == PlanKind::StreamPair;
+  if (p.kind == PlanKind::Pruned || p.kind == PlanKind::Pair || streamed) {
     HIP_TRY(c.sorted.reserve(n * 3 * sizeof(float)));
     HIP_TRY(c.perm.reserve(n * sizeof(uint32_t)));
     if (!c.spatial) c.spatial = new_spatial_scratch();
-    const uint32_t block =
-        (p.kind == PlanKind::StreamPruned) ? (uint32_t)kSBlock : (uint32_t)(64 * p.k->ppt);
+    const uint32_t block = streamed ? (uint32_t)kSBlock : (uint32_t)(64 * p.k->ppt);
     rc = spatial_order(d_pts, sp, ld, (uint32_t)n, block, (float *)c.sorted.p,
                        (uint32_t *)c.perm.p, *c.spatial, st);
     if (rc) return rc;
-    if (p.kind == PlanKind::StreamPruned) {
+    if (streamed) {
       // Min-distances by position; the first pass writes every one.
       HIP_TRY(c.dmin.reserve(n * sizeof(float)));
       a.dmin = (float *)c.dmin.p;
@@ -945,7 +959,7 @@ FPSX_EXPORT int fpsx_launch_exchanges(void *stream, uint64_t *exchanges) {
   std::lock_guard<std::mutex> lk(c.mu);
   *exchanges = 0;
   if (!c.last_plan.k || c.last_plan.grid < 2) return FPSX_OK;
-  if (c.last_plan.kind != PlanKind::Pair) {
+  if (c.last_plan.kind != PlanKind::Pair && c.last_plan.kind != PlanKind::StreamPair) {
     // One exchange per pick after the forced prefix.
     *exchanges = c.last_m > c.last_starts ? c.last_m - c.last_starts : 0;
     return FPSX_OK;

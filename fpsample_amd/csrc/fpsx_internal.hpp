@@ -27,7 +27,9 @@ struct DevBuf {
   ~DevBuf();
 };
 
-enum class PlanKind : int { Resident = 0, Stream = 1, Pruned = 2, Npdu = 3, StreamPruned = 4, Pair = 5 };
+enum class PlanKind : int {
+  Resident = 0, Stream = 1, Pruned = 2, Npdu = 3, StreamPruned = 4, Pair = 5, StreamPair = 6
+};
 
 struct Plan {
   PlanKind kind = PlanKind::Resident;

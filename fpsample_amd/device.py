@@ -417,5 +417,5 @@ def plan_info(n_points: int, dim: int, tie: int = TIE_HIGHEST, n_samples: int = 
     _raise(lib().fpsx_plan_info(n_points, dim, tie, n_samples, *[ctypes.byref(x) for x in v]),
            "fpsx_plan_info")
     kind, block, ppt, grid, chunk = (x.value for x in v)
-    return dict(kind=("resident", "stream", "pruned", "npdu", "spruned", "pair")[kind], block=block, ppt=ppt, grid=grid,
+    return dict(kind=("resident", "stream", "pruned", "npdu", "spruned", "pair", "spair")[kind], block=block, ppt=ppt, grid=grid,
                 chunk=chunk)

@@ -80,8 +80,9 @@ def lattice16m():
     return bench.gen_lattice_dups(1 << 24)
 
 
-@pytest.mark.parametrize("env", [{}, {"FPSX_PLAN": "stream:1024"}, {"FPSX_PLAN": "spruned:128"}],
-                         ids=["auto", "stream1024", "spruned128"])
+@pytest.mark.parametrize("env", [{}, {"FPSX_PLAN": "spruned"}, {"FPSX_PLAN": "stream:1024"},
+                                 {"FPSX_PLAN": "spruned:128"}],
+                         ids=["auto", "spruned", "stream1024", "spruned128"])
 def test_lattice16m_every_pick(gpu, monkeypatch, lattice16m, env):
     check_single(gpu, monkeypatch, "lattice16m", lattice16m, env)
 

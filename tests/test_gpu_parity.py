@@ -331,13 +331,14 @@ def test_accuracy_against_fp64_yardstick(gpu):
         assert acc["index_agreement"] >= 0.9, acc
 
 
-SPRUNED_PLANS = ["spruned", "spruned:1", "spruned:3", "spruned:16"]
+SPRUNED_PLANS = ["spruned", "spruned:1", "spruned:3", "spruned:16", "spair", "spair:2", "spair:3", "spair:16"]
 
 
 @pytest.mark.parametrize("plan", SPRUNED_PLANS)
 def test_stream_pruned_plans(gpu, plan_env, plan):
     # Streaming kernel with per-block pruning: points in HBM in spatial order,
-    # only blocks a pick can change are read.  Both tie rules, forced prefix.
+    # only blocks a pick can change are read; "spair" settles up to two picks
+    # per exchange.  Both tie rules, forced prefix.
     from fpsample_amd import device as D
 
     plan_env(plan)
@@ -361,8 +362,9 @@ def test_stream_pruned_plans(gpu, plan_env, plan):
         np.testing.assert_array_equal(out.numpy(), want, err_msg=f"{name} lowest")
 
 
-def test_stream_pruned_special_values(gpu, plan_env):
-    plan_env("spruned:4")
+@pytest.mark.parametrize("plan", ["spruned:4", "spair:4"])
+def test_stream_pruned_special_values(gpu, plan_env, plan):
+    plan_env(plan)
     rng = np.random.default_rng(42)
     pc = (rng.random((40000, 3)) * 4e19).astype(np.float32)
     pc[[5, 7000, 39999]] = np.nan
@@ -374,22 +376,49 @@ def test_stream_pruned_special_values(gpu, plan_env):
 
 
 def test_lattice_streaming_kernels_agree(gpu, plan_env):
-    # 16.7 M lattice points with 25 % duplicates, 4096 picks: the pruned
-    # streaming kernel (auto plan) and the plain streaming kernel must give the
-    # same sequence; the oracle checks the prefix.
+    # 16.7 M lattice points with 25 % duplicates, 4096 picks: the paired
+    # pruned streaming kernel (auto plan), the one-pick pruned streaming
+    # kernel and the plain streaming kernel must give the same sequence; the
+    # oracle checks the prefix.
     pc = c5_cloud()
     plan_env(None)
     info = gpu.plan_info(len(pc), 3, m=4096)
     log(f"C5 auto plan {info}")
-    assert info["kind"] == 4
+    assert info["kind"] == 6
     a = run_fps(gpu, pc, 4096, 0)
+    plan_env("spruned")
+    b = run_fps(gpu, pc, 4096, 0)
+    np.testing.assert_array_equal(a, b)
     plan_env("stream:1024")
     b = run_fps(gpu, pc, 4096, 0)
     np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(a[:24], O.fps(pc, 24, 0))
 
 
-@pytest.mark.parametrize("plan", ["pruned:1024:4", "pair:1024:4", "spruned:3"])
+@pytest.mark.parametrize("plan", ["spair:2", "spair:7"])
+def test_stream_pair_exhausts_cloud(gpu, plan_env, plan):
+    # Every point picked from a cloud of duplicates: the later rounds pick
+    # among zero min-distances, where pairs must fall back exactly as the
+    # reference's argmax does.  Both tie rules, a forced prefix.
+    from fpsample_amd import device as D
+
+    plan_env(plan)
+    rng = np.random.default_rng(43)
+    pc = np.repeat(rng.integers(0, 6, (1500, 3)), 3, axis=0).astype(np.float32)
+    n = len(pc)
+    log(f"spair exhaust {plan} n={n}")
+    np.testing.assert_array_equal(run_fps(gpu, pc, n, 0), O.fps(pc, n, 0))
+    np.testing.assert_array_equal(run_fps(gpu, pc, n, [4, 4, 17]), O.fps(pc, n, [4, 4, 17]))
+    soa = D.DeviceArray.from_numpy(np.ascontiguousarray(pc.T))
+    ds = D.DeviceArray.from_numpy(np.array([2], np.uint64))
+    out = D.DeviceArray((n,), np.uint64)
+    D.fps_sampling_device(soa, n, 3, n, ds, 1, out, tie=D.TIE_LOWEST)
+    D.synchronize()
+    D.device_status()
+    np.testing.assert_array_equal(out.numpy(), O.fps_ex(pc, n, 2, lowest=True), err_msg="lowest")
+
+
+@pytest.mark.parametrize("plan", ["pruned:1024:4", "pair:1024:4", "spruned:3", "spair:3"])
 def test_fused_gather_pruned_kinds(gpu, plan_env, plan):
     # The pruned, paired (both picks of a round) and streaming pruned kernels
     # write the picks' coordinates from their own pick records; two starts,
