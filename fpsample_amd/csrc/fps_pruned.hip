@@ -1497,7 +1497,7 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
   int *m_v = reinterpret_cast<int *>(m_hi + 3 * nbl);
   int *m_i = m_v + nbl;
   int *m_c = m_i + nbl;  // m_c[c * nbl + j]: coordinate bits of the block's best
-  int *s_list = m_c + 3 * nbl;  // blocks to stream (update, then wave 0's lookahead)
+  int *s_list = m_c + 3 * nbl;  // blocks to stream (update, then the lookahead)
   if (tid < 2) s_count[tid] = 0;
   lds_barrier();
   const float *xs = a.sorted;
@@ -1519,6 +1519,7 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
     ga[k] = 0;
     gl_[k] = 0;
   }
+  uint32_t ga_j = 0;  // A's block slot
   bool la_valid = false;
   int mu_v = INT_MIN;
   uint32_t mu_i = 0;
@@ -1676,6 +1677,7 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
         ga[1] = readlane_i((int)bi, gl);
 #pragma unroll
         for (int q = 0; q < D; ++q) ga[2 + q] = m_c[q * nbl + wj];
+        ga_j = wj;
         la_valid = false;
       }
       // Lookahead only while this workgroup could hold the next p1.
@@ -1683,7 +1685,7 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
       if (needL) la_valid = true;
       if (lane == 0) {
         s_a[0] = make_int4(ga[0], ga[1], ga[2], ga[3]);
-        s_a[1] = make_int4(ga[4], needL ? 1 : 0, 0, 0);
+        s_a[1] = make_int4(ga[4], needL ? 1 : 0, (int)ga_j, 0);
         s_cnt2 = 0;
       }
     }
@@ -1691,65 +1693,17 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
     if (s_a[1].y != 0) {
       // Lookahead, every wave.  L: the best cached candidate of the blocks A
       // leaves unchanged at their best (out of reach, or their best keeps its
-      // value); the other blocks are listed and streamed, one per wave.
+      // value); the other blocks are listed and streamed, one per wave.  A's
+      // own block always moves (its best is A) but for degenerate values:
+      // the last wave streams it, its loads issued while the other waves run
+      // the box tests (a kept one is also in L, as the same candidate).
       const int4 sa0 = s_a[0], sa1 = s_a[1];
       const float pa[D] = {__int_as_float(__builtin_amdgcn_readfirstlane(sa0.z)),
                            __int_as_float(__builtin_amdgcn_readfirstlane(sa0.w)),
                            __int_as_float(__builtin_amdgcn_readfirstlane(sa1.x))};
-      int lv = INT_MIN;
-      uint32_t li = 0;
-      int lc[D] = {0, 0, 0};
-      for (uint32_t j0 = (uint32_t)(wave * 64); j0 < nbl; j0 += 64u * W) {
-        const uint32_t jl = j0 + (uint32_t)lane;
-        bool sweep = false;
-        if (jl < nbl) {
-          const int v = m_v[jl];
-          const uint32_t id = (uint32_t)m_i[jl];
-          const float lo[D] = {m_lo[jl], m_lo[nbl + jl], m_lo[2 * nbl + jl]};
-          const float hi[D] = {m_hi[jl], m_hi[nbl + jl], m_hi[2 * nbl + jl]};
-          const int cb[D] = {m_c[jl], m_c[nbl + jl], m_c[2 * nbl + jl]};
-          const float cf[D] = {__int_as_float(cb[0]), __int_as_float(cb[1]), __int_as_float(cb[2])};
-          const float vf = __int_as_float(v);
-          const bool kept = !(box_bound<D>(pa, lo, hi) < vf) ||
-                            __float_as_int(min_keep(sqdist<D>(cf, pa), vf)) == v;
-          if (kept && beats<TIE>(v, id, lv, li)) {
-            lv = v;
-            li = id;
-#pragma unroll
-            for (int q = 0; q < D; ++q) lc[q] = cb[q];
-          }
-          sweep = !kept;
-        }
-        const uint64_t mask = __ballot(sweep);
-        if (mask != 0) {
-          int at = 0;
-          if (lane == 0) at = atomicAdd(&s_cnt2, __popcll(mask));
-          at = readlane_i(at, 0);
-          if (sweep) s_list[at + __popcll(mask & ((1ull << lane) - 1))] = (int)jl;
-        }
-      }
-      {
-        int Lv;
-        const int ll = wave_argmax_lane<TIE>(lv, li, Lv);
-        const int Li = readlane_i((int)li, ll);
-        int Lc[D];
-#pragma unroll
-        for (int q = 0; q < D; ++q) Lc[q] = readlane_i(lc[q], ll);
-        if (lane == 0) {
-          s_lp[wave][0] = make_int4(Lv, Li, Lc[0], Lc[1]);
-          s_lp[wave][1] = make_int4(Lc[2], 0, 0, 0);
-        }
-      }
-      lds_barrier();  // the list and every wave's L
-      const int n2 = s_cnt2;
-      int hv = INT_MIN;  // this lane's best of the streamed blocks
-      uint32_t hid = 0;
-      float hc[D] = {0.0f, 0.0f, 0.0f};
-      for (int u = wave; u < n2; u += W) {
-        const uint32_t j = (uint32_t)s_list[u];
+      const uint32_t ja = (uint32_t)__builtin_amdgcn_readfirstlane(sa1.z);
+      auto load_block = [&](uint32_t j, float(&bx)[SL][D], float(&bo)[SL], uint32_t(&bid)[SL]) {
         const uint32_t base = (g + j * G) * (uint32_t)kSBlock;
-        float bx[SL][D], bo[SL];
-        uint32_t bid[SL];
 #pragma unroll
         for (int k = 0; k < SL; ++k) {
           const uint32_t p = base + (uint32_t)(k * 64 + lane);
@@ -1762,6 +1716,13 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
           // what was there before is not a min-distance.
           bo[k] = first ? f.init : f.dmin[q];
         }
+      };
+      int hv = INT_MIN;  // this lane's best of the streamed blocks
+      uint32_t hid = 0;
+      float hc[D] = {0.0f, 0.0f, 0.0f};
+      auto sweep_block = [&](uint32_t j, const float(&bx)[SL][D], const float(&bo)[SL],
+                             const uint32_t(&bid)[SL]) {
+        const uint32_t base = (g + j * G) * (uint32_t)kSBlock;
 #pragma unroll
         for (int k = 0; k < SL; ++k) {
           const uint32_t p = base + (uint32_t)(k * 64 + lane);
@@ -1774,6 +1735,74 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
           hid = take ? bid[k] : hid;
 #pragma unroll
           for (int r = 0; r < D; ++r) hc[r] = take ? bx[k][r] : hc[r];
+        }
+      };
+      if (wave == W - 1) {
+        // A's own block: loads first, the barrier the others reach after
+        // their box tests, then the hypothetical update.
+        float ax[SL][D], ao[SL];
+        uint32_t aid[SL];
+        load_block(ja, ax, ao, aid);
+        if (lane == 0) {
+          s_lp[wave][0] = make_int4(INT_MIN, 0, 0, 0);
+          s_lp[wave][1] = make_int4(0, 0, 0, 0);
+        }
+        lds_barrier();  // the list and every wave's L
+        sweep_block(ja, ax, ao, aid);
+      } else {
+        // Box tests of slots [64 w, 64 w + 64), [64 w + 64 (W - 1), ...).
+        int lv = INT_MIN;
+        uint32_t li = 0;
+        int lc[D] = {0, 0, 0};
+        for (uint32_t j0 = (uint32_t)(wave * 64); j0 < nbl; j0 += 64u * (W - 1)) {
+          const uint32_t jl = j0 + (uint32_t)lane;
+          bool sweep = false;
+          if (jl < nbl) {
+            const int v = m_v[jl];
+            const uint32_t id = (uint32_t)m_i[jl];
+            const float lo[D] = {m_lo[jl], m_lo[nbl + jl], m_lo[2 * nbl + jl]};
+            const float hi[D] = {m_hi[jl], m_hi[nbl + jl], m_hi[2 * nbl + jl]};
+            const int cb[D] = {m_c[jl], m_c[nbl + jl], m_c[2 * nbl + jl]};
+            const float cf[D] = {__int_as_float(cb[0]), __int_as_float(cb[1]), __int_as_float(cb[2])};
+            const float vf = __int_as_float(v);
+            const bool kept = !(box_bound<D>(pa, lo, hi) < vf) ||
+                              __float_as_int(min_keep(sqdist<D>(cf, pa), vf)) == v;
+            if (kept && beats<TIE>(v, id, lv, li)) {
+              lv = v;
+              li = id;
+#pragma unroll
+              for (int q = 0; q < D; ++q) lc[q] = cb[q];
+            }
+            sweep = !kept && jl != ja;
+          }
+          const uint64_t mask = __ballot(sweep);
+          if (mask != 0) {
+            int at = 0;
+            if (lane == 0) at = atomicAdd(&s_cnt2, __popcll(mask));
+            at = readlane_i(at, 0);
+            if (sweep) s_list[at + __popcll(mask & ((1ull << lane) - 1))] = (int)jl;
+          }
+        }
+        {
+          int Lv;
+          const int ll = wave_argmax_lane<TIE>(lv, li, Lv);
+          const int Li = readlane_i((int)li, ll);
+          int Lc[D];
+#pragma unroll
+          for (int q = 0; q < D; ++q) Lc[q] = readlane_i(lc[q], ll);
+          if (lane == 0) {
+            s_lp[wave][0] = make_int4(Lv, Li, Lc[0], Lc[1]);
+            s_lp[wave][1] = make_int4(Lc[2], 0, 0, 0);
+          }
+        }
+        lds_barrier();  // the list and every wave's L
+        const int n2 = s_cnt2;
+        for (int u = wave; u < n2; u += W - 1) {
+          const uint32_t j = (uint32_t)s_list[u];
+          float bx[SL][D], bo[SL];
+          uint32_t bid[SL];
+          load_block(j, bx, bo, bid);
+          sweep_block(j, bx, bo, bid);
         }
       }
       {
