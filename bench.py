@@ -397,6 +397,10 @@ def roofline(workload: str, plan: dict, n: int, m: int, kernel_s: float, npdu: b
                exchanges_per_launch=exchanges, exchange_floor_us=round(floor_us, 4),
                us_per_exchange=round(kernel_s * 1e6 / exchanges, 4),
                floor_probe={"grid": grid, "words": words, "threads": 1024})
+    if kind in ("spruned", "spair"):
+        out["note"] = ("the floor is the exchange alone; a streamed round also waits for the memory "
+                       "round trip of its listed blocks (a paired round for two: the update's, then "
+                       "the lookahead's)")
     return out
 
 
