@@ -66,7 +66,7 @@ hipError_t DevBuf::reserve(size_t bytes) {
 }
 
 void DevBuf::free() {
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);  // release path: nothing to report to
   p = nullptr;
   cap = 0;
 }
@@ -1060,8 +1060,8 @@ FPSX_EXPORT int fpsx_autotune(const float *d_points, size_t stride_point, size_t
     if (*c.host_status != 0) continue;  // a candidate that failed is never chosen
     times[ci] = tmin;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);  // timing events only
+  (void)hipEventDestroy(e1);
   // Within 3% of the fastest, prefer the fewest workgroups: the choice is
   // then stable from run to run, and fewer workgroups exchange less.
   const double best = *std::min_element(times.begin(), times.end());

@@ -340,8 +340,8 @@ FPSX_EXPORT int fpsx_batch_create(const float *d_points, size_t ld, size_t dim,
         chosen_pruned = cd.pruned;
       }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);  // timing events only
+    (void)hipEventDestroy(e1);
   }
   if (!chosen) {
     // Heuristic: the first pruned kernel whose teams hold every cloud; else
@@ -461,9 +461,10 @@ FPSX_EXPORT int fpsx_batch_info(fpsx_batch *b, int *block, int *ppt, int *grid, 
 
 FPSX_EXPORT void fpsx_batch_destroy(fpsx_batch *b) {
   if (!b) return;
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
-  if (b->capture_stream) hipStreamDestroy(b->capture_stream);
+  // Release path: a failed destroy leaves nothing for the caller to do.
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   b->sorted.free();
   b->perm.free();
   delete_spatial_scratch(b->spatial);

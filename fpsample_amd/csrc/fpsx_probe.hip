@@ -101,7 +101,7 @@ FPSX_EXPORT int fpsx_probe_exchange(unsigned grid, unsigned words, unsigned roun
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return fail(FPSX_EHIP, "probe: hipEventCreate");
   if (hipEventCreate(&e1) != hipSuccess) {
-    hipEventDestroy(e0);
+    (void)hipEventDestroy(e0);
     return fail(FPSX_EHIP, "probe: hipEventCreate");
   }
   void *args[] = {&a};
@@ -111,8 +111,8 @@ FPSX_EXPORT int fpsx_probe_exchange(unsigned grid, unsigned words, unsigned roun
   if (err == hipSuccess) err = hipEventSynchronize(e1);
   float ms = 0.0f;
   if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (err != hipSuccess) return fail(FPSX_EHIP, std::string("probe: ") + hipGetErrorString(err));
   rc = scratch_release(c, st);
   if (rc) return rc;

@@ -340,7 +340,7 @@ int fpsx_server::step() {
           continue;
         }
         arena->release(r->off, r->bytes);
-        if (r->uploaded) hipEventDestroy(r->uploaded);
+        if (r->uploaded) (void)hipEventDestroy(r->uploaded);
         r->uploaded = nullptr;
         active.erase(std::find(active.begin(), active.end(), r));
         end_request(*r, r->stop_code, Clock::now());
@@ -396,7 +396,7 @@ int fpsx_server::step() {
     if (e != hipSuccess) return fail(FPSX_EHIP, std::string("evict: ") + hipGetErrorString(e));
     std::lock_guard<std::mutex> lk(mu);
     arena->release(r->off, r->bytes);
-    if (r->uploaded) hipEventDestroy(r->uploaded);
+    if (r->uploaded) (void)hipEventDestroy(r->uploaded);
     r->uploaded = nullptr;
     r->ready = false;
     r->evict = false;
@@ -490,7 +490,7 @@ int fpsx_server::finish(Launch &L) {
       r->done = true;
       r->t_done = now;
       arena->release(r->off, r->bytes);
-      if (r->uploaded) hipEventDestroy(r->uploaded);
+      if (r->uploaded) (void)hipEventDestroy(r->uploaded);
       r->uploaded = nullptr;
       active.erase(std::find(active.begin(), active.end(), r));
       ++completed;
@@ -502,14 +502,19 @@ int fpsx_server::finish(Launch &L) {
 }
 
 void fpsx_server::loop() {
-  hipSetDevice(ctx->device);
+  // Without its device this thread can run nothing: every request fails.
+  int rc = hipSetDevice(ctx->device) == hipSuccess
+               ? FPSX_OK
+               : fail(FPSX_EHIP, "server thread: hipSetDevice failed");
   for (;;) {
-    {
-      std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
-      if (stop && pending.empty() && active.empty()) return;
+    if (rc == FPSX_OK) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
+        if (stop && pending.empty() && active.empty()) return;
+      }
+      rc = step();
     }
-    const int rc = step();
     if (rc != FPSX_OK) {
       std::lock_guard<std::mutex> lk(mu);
       fatal = rc;
@@ -517,7 +522,7 @@ void fpsx_server::loop() {
       for (auto &r : active) {
         r->done = true;
         r->error = rc;
-        if (r->uploaded) hipEventDestroy(r->uploaded);
+        if (r->uploaded) (void)hipEventDestroy(r->uploaded);
         r->uploaded = nullptr;
       }
       for (auto &r : pending) {
@@ -572,7 +577,7 @@ FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passe
                          (uint32_t)(64 * P));
     if (rc) {
       delete_spatial_scratch(s->spatial);
-      hipFree(s->arena_base);
+      (void)hipFree(s->arena_base);
       return rc;
     }
   }
@@ -580,9 +585,9 @@ FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passe
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking);
   if (e != hipSuccess) {
     const std::string msg = "streams: " + std::string(hipGetErrorString(e));
-    if (s->compute) hipStreamDestroy(s->compute);
+    if (s->compute) (void)hipStreamDestroy(s->compute);
     delete_spatial_scratch(s->spatial);
-    hipFree(s->arena_base);
+    (void)hipFree(s->arena_base);
     return fail(FPSX_EHIP, msg);
   }
   for (auto &L : s->slot) {
@@ -591,10 +596,10 @@ FPSX_EXPORT int fpsx_server_create(size_t hbm_budget_bytes, unsigned chunk_passe
   }
   if (e != hipSuccess) {
     const std::string msg = "events: " + std::string(hipGetErrorString(e));
-    hipStreamDestroy(s->compute);
-    hipStreamDestroy(s->copy);
+    (void)hipStreamDestroy(s->compute);
+    (void)hipStreamDestroy(s->copy);
     delete_spatial_scratch(s->spatial);
-    hipFree(s->arena_base);
+    (void)hipFree(s->arena_base);
     return fail(FPSX_EHIP, msg);
   }
   fpsx_server *raw = s.get();
@@ -751,20 +756,22 @@ FPSX_EXPORT void fpsx_server_destroy(fpsx_server *s) {
   }
   s->cv_work.notify_all();
   if (s->thread.joinable()) s->thread.join();
-  hipStreamSynchronize(s->compute);
-  hipStreamSynchronize(s->copy);
+  // Destroy path (no error can be returned): the streams drain before their
+  // memory is released.
+  (void)hipStreamSynchronize(s->compute);
+  (void)hipStreamSynchronize(s->copy);
   for (auto &L : s->slot) {
     L.run.d_jobs.free();
     L.run.d_member_job.free();
     L.run.d_counters.free();
     L.run.d_xchg.free();
     L.run.d_status.free();
-    if (L.t0) hipEventDestroy(L.t0);
-    if (L.t1) hipEventDestroy(L.t1);
+    if (L.t0) (void)hipEventDestroy(L.t0);
+    if (L.t1) (void)hipEventDestroy(L.t1);
   }
-  if (s->arena_base) hipFree(s->arena_base);
+  if (s->arena_base) (void)hipFree(s->arena_base);
   delete_spatial_scratch(s->spatial);
-  hipStreamDestroy(s->compute);
-  hipStreamDestroy(s->copy);
+  (void)hipStreamDestroy(s->compute);
+  (void)hipStreamDestroy(s->copy);
   delete s;
 }
