@@ -1476,12 +1476,17 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
   constexpr int NP = 2 * NW;
   constexpr int SL = kSBlock / 64;  // slots per lane in a block
   __shared__ int4 s_res[2][3];      // p1, p2, two-pick flag, all-arrived flag
-  extern __shared__ int s_meta[];   // kSMetaWords rows of nbl words, then the block list
+  extern __shared__ int s_meta[];   // kSPairMetaWords rows of nbl words
   __shared__ int s_count[2];        // listed blocks, by pass parity
   __shared__ int4 s_a[2];           // A (value, index, coordinates), lookahead flag
   __shared__ int s_cnt2;            // blocks the lookahead streams
   __shared__ int4 s_lp[W][2];       // per wave: best kept candidate (L)
-  __shared__ int4 s_hp[W][2];       // per wave: best of its streamed blocks
+  __shared__ int4 s_hp[W][2];       // per wave: best of its streamed rows or blocks
+  __shared__ float4 s_stg[SL][64];  // the staged block: x, y, z, min-distance
+  __shared__ uint32_t s_stg_id[SL][64];
+  __shared__ int s_spec;            // slot of the staged block, or -1
+  __shared__ int4 s_sb[2];          // best of the blocks not updated (valid when s_spec >= 0)
+  static_assert(SL == W / 2, "the second half of the waves streams A's block, one row each");
 
   const FpsParams &f = a.fps;
   const int tid = threadIdx.x;
@@ -1498,6 +1503,8 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
   int *m_i = m_v + nbl;
   int *m_c = m_i + nbl;  // m_c[c * nbl + j]: coordinate bits of the block's best
   int *s_list = m_c + 3 * nbl;  // blocks to stream (update, then the lookahead)
+  uint32_t *m_mark = reinterpret_cast<uint32_t *>(s_list + nbl);  // pass that last listed the slot
+  for (uint32_t j = (uint32_t)tid; j < nbl; j += B) m_mark[j] = 0u;
   if (tid < 2) s_count[tid] = 0;
   lds_barrier();
   const float *xs = a.sorted;
@@ -1555,7 +1562,10 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
         int at = 0;
         if (lane == 0) at = atomicAdd(&s_count[sp], __popcll(mask));
         at = readlane_i(at, 0);
-        if (need) s_list[at + __popcll(mask & ((1ull << lane) - 1))] = (int)jl;
+        if (need) {
+          s_list[at + __popcll(mask & ((1ull << lane) - 1))] = (int)jl;
+          m_mark[jl] = t;
+        }
       }
     }
     lds_barrier();  // the list is complete
@@ -1632,6 +1642,69 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
         }
       }
     }
+    // Speculation (the last wave, when the update leaves it idle): the best
+    // cached candidate of the blocks this pass does not update.  It is the
+    // workgroup's A after the update unless an updated block's new best beats
+    // it, and a lookahead streams A's own block, so that block's points and
+    // min-distances (current: it is not written this pass) are staged in LDS
+    // while the update's memory round trip runs.  The lookahead uses them
+    // only if A's block is the staged one.
+    if (wave == W - 1) {
+      int sj = -1;
+      if (!first && s >= f.n_starts && listed > 0 && listed < W) {
+        int bv = INT_MIN;
+        uint32_t bi = 0, bj = 0;
+        for (uint32_t j0 = 0; j0 < nbl; j0 += 64u * 8u) {
+          int v[8];
+          uint32_t id[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const uint32_t jl = j0 + (uint32_t)(u * 64 + lane);
+            const uint32_t jc = jl < nbl ? jl : nbl - 1u;
+            v[u] = m_v[jc];
+            id[u] = (uint32_t)m_i[jc];
+            const bool skip = jl >= nbl || m_mark[jc] == t;  // padding, or updated this pass
+            v[u] = skip ? INT_MIN : v[u];
+            id[u] = skip ? 0u : id[u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const bool take = beats<TIE>(v[u], id[u], bv, bi);
+            bv = take ? v[u] : bv;
+            bi = take ? id[u] : bi;
+            bj = take ? j0 + (uint32_t)(u * 64 + lane) : bj;
+          }
+        }
+        int gv;
+        const int gl = wave_argmax_lane<TIE>(bv, bi, gv);
+        if (gv != INT_MIN) {
+          const uint32_t tj = (uint32_t)readlane_i((int)bj, gl);
+          sj = (int)tj;
+          if (lane == 0) {
+            // The best cached candidate of the blocks not updated this pass:
+            // wave 0 then forms A from it and the updated blocks alone.
+            s_sb[0] = make_int4(gv, readlane_i((int)bi, gl), (int)tj, m_c[tj]);
+            s_sb[1] = make_int4(m_c[nbl + tj], m_c[2 * nbl + tj], 0, 0);
+          }
+          const uint32_t base = (g + tj * G) * (uint32_t)kSBlock;
+          float4 r[SL];
+          uint32_t rid[SL];
+#pragma unroll
+          for (int k = 0; k < SL; ++k) {
+            const uint32_t p = base + (uint32_t)(k * 64 + lane);
+            const uint32_t q = p < n ? p : n - 1;
+            r[k] = make_float4(xs[q], ys[q], zs[q], f.dmin[q]);
+            rid[k] = a.perm[q];
+          }
+#pragma unroll
+          for (int k = 0; k < SL; ++k) {
+            s_stg[k][lane] = r[k];
+            s_stg_id[k][lane] = rid[k];
+          }
+        }
+      }
+      if (lane == 0) s_spec = sj;
+    }
     lds_barrier();  // every block's cached candidate is current
     if (s < f.n_starts) {
       // Forced prefix (multi-start): one pick per pass, no exchange.
@@ -1646,39 +1719,73 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
     const int par = (int)(e & 1u);
     if (wave == 0) {
       if (e == 1u || listed > 0) {
-        // The workgroup's A: argmax over the cached candidates, eight slots
-        // per lane loaded at once.
-        int bv = INT_MIN;
-        uint32_t bi = 0, bj = 0;
-        for (uint32_t j0 = 0; j0 < nbl; j0 += 64u * 8u) {
-          int v[8];
-          uint32_t id[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const uint32_t jl = j0 + (uint32_t)(u * 64 + lane);
-            const uint32_t jc = jl < nbl ? jl : nbl - 1u;
-            v[u] = m_v[jc];
-            id[u] = (uint32_t)m_i[jc];
-            v[u] = jl < nbl ? v[u] : INT_MIN;  // padding never beats a real slot
-            id[u] = jl < nbl ? id[u] : 0u;
+        if (s_spec >= 0) {
+          // A from the speculation's best of the blocks not updated (lane 0)
+          // and the updated blocks (lanes 1..listed, listed < W).
+          const int4 b0 = s_sb[0];
+          const uint32_t jl = lane >= 1 && lane <= listed ? (uint32_t)s_list[lane - 1] : 0u;
+          int v = INT_MIN;
+          uint32_t id = 0, js = 0;
+          if (lane == 0) {
+            v = b0.x;
+            id = (uint32_t)b0.y;
+            js = (uint32_t)b0.z;
+          } else if (lane <= listed) {
+            v = m_v[jl];
+            id = (uint32_t)m_i[jl];
+            js = jl;
           }
+          int gv;
+          const int gl = wave_argmax_lane<TIE>(v, id, gv);
+          const uint32_t wj = (uint32_t)readlane_i((int)js, gl);
+          ga[0] = gv;
+          ga[1] = readlane_i((int)id, gl);
+          if (gl == 0) {
+            const int4 b1 = s_sb[1];
+            ga[2] = b0.w;
+            ga[3] = b1.x;
+            ga[4] = b1.y;
+          } else {
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const bool take = beats<TIE>(v[u], id[u], bv, bi);
-            bv = take ? v[u] : bv;
-            bi = take ? id[u] : bi;
-            bj = take ? j0 + (uint32_t)(u * 64 + lane) : bj;
+            for (int q = 0; q < D; ++q) ga[2 + q] = m_c[q * nbl + wj];
           }
+          ga_j = wj;
+          la_valid = false;
+        } else {
+          // The workgroup's A: argmax over the cached candidates, eight slots
+          // per lane loaded at once.
+          int bv = INT_MIN;
+          uint32_t bi = 0, bj = 0;
+          for (uint32_t j0 = 0; j0 < nbl; j0 += 64u * 8u) {
+            int v[8];
+            uint32_t id[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const uint32_t jl = j0 + (uint32_t)(u * 64 + lane);
+              const uint32_t jc = jl < nbl ? jl : nbl - 1u;
+              v[u] = m_v[jc];
+              id[u] = (uint32_t)m_i[jc];
+              v[u] = jl < nbl ? v[u] : INT_MIN;  // padding never beats a real slot
+              id[u] = jl < nbl ? id[u] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const bool take = beats<TIE>(v[u], id[u], bv, bi);
+              bv = take ? v[u] : bv;
+              bi = take ? id[u] : bi;
+              bj = take ? j0 + (uint32_t)(u * 64 + lane) : bj;
+            }
+          }
+          int gv;
+          const int gl = wave_argmax_lane<TIE>(bv, bi, gv);
+          const uint32_t wj = (uint32_t)readlane_i((int)bj, gl);
+          ga[0] = gv;
+          ga[1] = readlane_i((int)bi, gl);
+#pragma unroll
+          for (int q = 0; q < D; ++q) ga[2 + q] = m_c[q * nbl + wj];
+          ga_j = wj;
+          la_valid = false;
         }
-        int gv;
-        const int gl = wave_argmax_lane<TIE>(bv, bi, gv);
-        const uint32_t wj = (uint32_t)readlane_i((int)bj, gl);
-        ga[0] = gv;
-        ga[1] = readlane_i((int)bi, gl);
-#pragma unroll
-        for (int q = 0; q < D; ++q) ga[2 + q] = m_c[q * nbl + wj];
-        ga_j = wj;
-        la_valid = false;
       }
       // Lookahead only while this workgroup could hold the next p1.
       const bool needL = !la_valid && beats<TIE>(ga[0], (uint32_t)ga[1], mu_v, mu_i);
@@ -1691,70 +1798,83 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
     }
     lds_barrier();  // A and the lookahead decision
     if (s_a[1].y != 0) {
-      // Lookahead, every wave.  L: the best cached candidate of the blocks A
-      // leaves unchanged at their best (out of reach, or their best keeps its
-      // value); the other blocks are listed and streamed, one per wave.  A's
-      // own block always moves (its best is A) but for degenerate values:
-      // the last wave streams it, its loads issued while the other waves run
-      // the box tests (a kept one is also in L, as the same candidate).
+      // Lookahead.  L: the best cached candidate of the blocks A leaves
+      // unchanged at their best (out of reach, or their best keeps its
+      // value): waves 0..7 test the blocks and list the others.  A's own
+      // block always moves (its best is A) but for degenerate values: waves
+      // 8..15 stream it meanwhile, one row of 64 slots each, from the staged
+      // copy when the speculation found it (a kept one is also in L, as the
+      // same candidate).  The listed blocks, if any, are streamed after that.
+      // A block written this pass gets this pass's pivots again (its stores
+      // may not have landed; min'ing twice changes nothing), and in the first
+      // pass every min-distance starts from the initial value.
       const int4 sa0 = s_a[0], sa1 = s_a[1];
       const float pa[D] = {__int_as_float(__builtin_amdgcn_readfirstlane(sa0.z)),
                            __int_as_float(__builtin_amdgcn_readfirstlane(sa0.w)),
                            __int_as_float(__builtin_amdgcn_readfirstlane(sa1.x))};
       const uint32_t ja = (uint32_t)__builtin_amdgcn_readfirstlane(sa1.z);
-      auto load_block = [&](uint32_t j, float(&bx)[SL][D], float(&bo)[SL], uint32_t(&bid)[SL]) {
-        const uint32_t base = (g + j * G) * (uint32_t)kSBlock;
-#pragma unroll
-        for (int k = 0; k < SL; ++k) {
-          const uint32_t p = base + (uint32_t)(k * 64 + lane);
-          const uint32_t q = p < n ? p : n - 1;
-          bx[k][0] = xs[q];
-          bx[k][1] = ys[q];
-          bx[k][2] = zs[q];
-          bid[k] = a.perm[q];
-          // First pass: the stores of this pass may not have landed, and
-          // what was there before is not a min-distance.
-          bo[k] = first ? f.init : f.dmin[q];
-        }
-      };
-      int hv = INT_MIN;  // this lane's best of the streamed blocks
+      int hv = INT_MIN;  // this lane's best of the streamed slots
       uint32_t hid = 0;
       float hc[D] = {0.0f, 0.0f, 0.0f};
-      auto sweep_block = [&](uint32_t j, const float(&bx)[SL][D], const float(&bo)[SL],
-                             const uint32_t(&bid)[SL]) {
-        const uint32_t base = (g + j * G) * (uint32_t)kSBlock;
+      // Folds slot (c, min-distance d0, original index id) under the pivots.
+      auto sweep_slot = [&](bool live, const float(&c)[D], float d0, uint32_t id, bool again) {
+        float h = d0;
+        if (again) {
+          h = min_keep(sqdist<D>(c, p1), h);
+          if (have2) h = min_keep(sqdist<D>(c, p2), h);
+        }
+        h = min_keep(sqdist<D>(c, pa), h);
+        const int b = __float_as_int(h);
+        const bool take = live && beats<TIE>(b, id, hv, hid);
+        hv = take ? b : hv;
+        hid = take ? id : hid;
 #pragma unroll
-        for (int k = 0; k < SL; ++k) {
-          const uint32_t p = base + (uint32_t)(k * 64 + lane);
-          float h = min_keep(sqdist<D>(bx[k], p1), bo[k]);
-          if (have2) h = min_keep(sqdist<D>(bx[k], p2), h);
-          h = min_keep(sqdist<D>(bx[k], pa), h);
-          const int b = __float_as_int(h);
-          const bool take = p < n && beats<TIE>(b, bid[k], hv, hid);
-          hv = take ? b : hv;
-          hid = take ? bid[k] : hid;
+        for (int r = 0; r < D; ++r) hc[r] = take ? c[r] : hc[r];
+      };
+      auto publish_h = [&]() {
+        int Hv;
+        const int hl = wave_argmax_lane<TIE>(hv, hid, Hv);
+        const int Hi = readlane_i((int)hid, hl);
+        int Hc[D];
 #pragma unroll
-          for (int r = 0; r < D; ++r) hc[r] = take ? bx[k][r] : hc[r];
+        for (int q = 0; q < D; ++q) Hc[q] = __float_as_int(readlane_f(hc[q], hl));
+        if (lane == 0) {
+          s_hp[wave][0] = make_int4(Hv, Hi, Hc[0], Hc[1]);
+          s_hp[wave][1] = make_int4(Hc[2], 0, 0, 0);
         }
       };
-      if (wave == W - 1) {
-        // A's own block: loads first, the barrier the others reach after
-        // their box tests, then the hypothetical update.
-        float ax[SL][D], ao[SL];
-        uint32_t aid[SL];
-        load_block(ja, ax, ao, aid);
-        if (lane == 0) {
-          s_lp[wave][0] = make_int4(INT_MIN, 0, 0, 0);
-          s_lp[wave][1] = make_int4(0, 0, 0, 0);
+      if (wave >= W / 2) {
+        // Row (wave - W/2) of A's block.
+        const int k = wave - W / 2;
+        const uint32_t p = (g + ja * G) * (uint32_t)kSBlock + (uint32_t)(k * 64 + lane);
+        float c[D], d0;
+        uint32_t id;
+        bool again;
+        if (s_spec == (int)ja) {
+          const float4 r = s_stg[k][lane];
+          c[0] = r.x;
+          c[1] = r.y;
+          c[2] = r.z;
+          d0 = r.w;
+          id = s_stg_id[k][lane];
+          again = false;  // staged after this pass's stores were ordered; not written since
+        } else {
+          const uint32_t q = p < n ? p : n - 1;
+          c[0] = xs[q];
+          c[1] = ys[q];
+          c[2] = zs[q];
+          id = a.perm[q];
+          d0 = first ? f.init : f.dmin[q];
+          again = first || m_mark[ja] == t;
         }
-        lds_barrier();  // the list and every wave's L
-        sweep_block(ja, ax, ao, aid);
+        sweep_slot(p < n, c, d0, id, again);
+        publish_h();
       } else {
-        // Box tests of slots [64 w, 64 w + 64), [64 w + 64 (W - 1), ...).
+        // Box tests of slots [64 w, 64 w + 64), [64 w + 64 W/2, ...).
         int lv = INT_MIN;
         uint32_t li = 0;
         int lc[D] = {0, 0, 0};
-        for (uint32_t j0 = (uint32_t)(wave * 64); j0 < nbl; j0 += 64u * (W - 1)) {
+        for (uint32_t j0 = (uint32_t)(wave * 64); j0 < nbl; j0 += 64u * (W / 2)) {
           const uint32_t jl = j0 + (uint32_t)lane;
           bool sweep = false;
           if (jl < nbl) {
@@ -1783,47 +1903,55 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
             if (sweep) s_list[at + __popcll(mask & ((1ull << lane) - 1))] = (int)jl;
           }
         }
-        {
-          int Lv;
-          const int ll = wave_argmax_lane<TIE>(lv, li, Lv);
-          const int Li = readlane_i((int)li, ll);
-          int Lc[D];
+        int Lv;
+        const int ll = wave_argmax_lane<TIE>(lv, li, Lv);
+        const int Li = readlane_i((int)li, ll);
+        int Lc[D];
 #pragma unroll
-          for (int q = 0; q < D; ++q) Lc[q] = readlane_i(lc[q], ll);
-          if (lane == 0) {
-            s_lp[wave][0] = make_int4(Lv, Li, Lc[0], Lc[1]);
-            s_lp[wave][1] = make_int4(Lc[2], 0, 0, 0);
-          }
-        }
-        lds_barrier();  // the list and every wave's L
-        const int n2 = s_cnt2;
-        for (int u = wave; u < n2; u += W - 1) {
-          const uint32_t j = (uint32_t)s_list[u];
-          float bx[SL][D], bo[SL];
-          uint32_t bid[SL];
-          load_block(j, bx, bo, bid);
-          sweep_block(j, bx, bo, bid);
-        }
-      }
-      {
-        int Hv;
-        const int hl = wave_argmax_lane<TIE>(hv, hid, Hv);
-        const int Hi = readlane_i((int)hid, hl);
-        int Hc[D];
-#pragma unroll
-        for (int q = 0; q < D; ++q) Hc[q] = __float_as_int(readlane_f(hc[q], hl));
+        for (int q = 0; q < D; ++q) Lc[q] = readlane_i(lc[q], ll);
         if (lane == 0) {
-          s_hp[wave][0] = make_int4(Hv, Hi, Hc[0], Hc[1]);
-          s_hp[wave][1] = make_int4(Hc[2], 0, 0, 0);
+          s_lp[wave][0] = make_int4(Lv, Li, Lc[0], Lc[1]);
+          s_lp[wave][1] = make_int4(Lc[2], 0, 0, 0);
+          s_hp[wave][0] = make_int4(INT_MIN, 0, 0, 0);  // no streamed block yet
+          s_hp[wave][1] = make_int4(0, 0, 0, 0);
         }
       }
-      lds_barrier();  // every wave's streamed best
+      lds_barrier();  // every wave's L and A-row best; the list
+      const int n2 = s_cnt2;
+      if (n2 > 0) {
+        // The listed blocks, waves 0..7 in turn.
+        if (wave < W / 2) {
+          for (int u = wave; u < n2; u += W / 2) {
+            const uint32_t j = (uint32_t)s_list[u];
+            const uint32_t base = (g + j * G) * (uint32_t)kSBlock;
+            const bool again = first || m_mark[j] == t;
+            float bx[SL][D], bo[SL];
+            uint32_t bid[SL];
+#pragma unroll
+            for (int k = 0; k < SL; ++k) {
+              const uint32_t p = base + (uint32_t)(k * 64 + lane);
+              const uint32_t q = p < n ? p : n - 1;
+              bx[k][0] = xs[q];
+              bx[k][1] = ys[q];
+              bx[k][2] = zs[q];
+              bid[k] = a.perm[q];
+              bo[k] = first ? f.init : f.dmin[q];
+            }
+#pragma unroll
+            for (int k = 0; k < SL; ++k)
+              sweep_slot(base + (uint32_t)(k * 64 + lane) < n, bx[k], bo[k], bid[k], again);
+          }
+          publish_h();
+        }
+        lds_barrier();  // every wave's streamed best
+      }
       if (wave == 0) {
         // A' = the best of the waves' L's and streamed bests, one per lane.
-        const bool has = lane < 2 * W;
-        const int r = has ? (lane < W ? lane : lane - W) : 0;
-        const int4 c0 = lane < W ? s_lp[r][0] : s_hp[r][0];
-        const int c4 = lane < W ? s_lp[r][1].x : s_hp[r][1].x;
+        const bool has = lane < W / 2 + W;
+        const bool isl = lane < W / 2;
+        const int r = has ? (isl ? lane : lane - W / 2) : 0;
+        const int4 c0 = isl ? s_lp[r][0] : s_hp[r][0];
+        const int c4 = isl ? s_lp[r][1].x : s_hp[r][1].x;
         int av;
         const int al = wave_argmax_lane<TIE>(has ? c0.x : INT_MIN, (uint32_t)c0.y, av);
         gl_[0] = av;
