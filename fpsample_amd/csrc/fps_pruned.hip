@@ -1543,8 +1543,10 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
     // This wave's min-distance stores of the previous pass complete before
     // the barrier below, after which any wave may read those blocks.
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    // Box tests against the pending pivots.
-    for (uint32_t j0 = (uint32_t)(wave * 64); j0 < nbl; j0 += 64u * W) {
+    // Box tests against the pending pivots: wave w takes slots from
+    // 64 ((w + W - 1) mod W) on, so that wave 0, which is forming M, has
+    // none for up to 960 blocks.
+    for (uint32_t j0 = (uint32_t)(((wave + W - 1) % W) * 64); j0 < nbl; j0 += 64u * W) {
       const uint32_t jl = j0 + (uint32_t)lane;
       bool need = false;
       if (jl < nbl) {
@@ -1961,6 +1963,8 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
         gl_[4] = readlane_i(c4, al);
       }
     }
+    int mc_v = INT_MIN;  // wave 0: this lane's workgroup's A if it counts for M
+    uint32_t mc_i = 0;
     if (wave == 0) {
       // Exchange: publish the pair to K replicas, poll replica g mod K, one
       // workgroup per lane, all NP words at once.
@@ -2049,22 +2053,27 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
         s_res[par][2] = make_int4(r2[1], r2[2], two ? 1 : 0, all_ok ? 1 : 0);
       }
       // M for the next pass: the best A among the other workgroups whose A
-      // keeps its value under this pass's picks (so stays their best).
+      // keeps its value under this pass's picks (so stays their best);
+      // reduced after the hand-off, while the other waves start the next
+      // pass's box tests (wave 0 has none of its own there).
       const bool keep1 = __float_as_int(post) == x[0];
       const float post2 = min_keep(sqdist<D>(qa, q2), __int_as_float(x[0]));
       const bool keep = valid && x[0] != INT_MIN && keep1 &&
                         !(two && __float_as_int(post2) != x[0]);
-      int mv;
-      const int ml = wave_argmax_lane<TIE>((keep && (uint32_t)lane != g) ? x[0] : INT_MIN,
-                                           (uint32_t)x[1], mv);
-      mu_v = mv;
-      mu_i = (uint32_t)readlane_i(x[1], ml);
+      mc_v = (keep && (uint32_t)lane != g) ? x[0] : INT_MIN;
+      mc_i = (uint32_t)x[1];
     }
     lds_barrier();
     const int4 q0 = s_res[par][0], qq1 = s_res[par][1], qq2 = s_res[par][2];
     if (qq2.w == 0) {
       if (tid == 0) atomicOr(f.status, kStatusTimeout);
       return;
+    }
+    if (wave == 0) {
+      int mv;
+      const int ml = wave_argmax_lane<TIE>(mc_v, mc_i, mv);
+      mu_v = mv;
+      mu_i = (uint32_t)readlane_i((int)mc_i, ml);
     }
     i1 = (uint32_t)__builtin_amdgcn_readfirstlane(q0.y);
     p1[0] = __int_as_float(__builtin_amdgcn_readfirstlane(q0.z));
