@@ -1541,8 +1541,9 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
     const int sp = (int)(t & 1u);
     ++t;
     // This wave's min-distance stores of the previous pass complete before
-    // the barrier below, after which any wave may read those blocks.
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // the barrier below, after which any wave may read those blocks (wave 0
+    // writes none).
+    if (wave != 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     // Box tests against the pending pivots: wave w takes slots from
     // 64 ((w + W - 1) mod W) on, so that wave 0, which is forming M, has
     // none for up to 960 blocks.
@@ -1573,8 +1574,11 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
     lds_barrier();  // the list is complete
     if (tid == 0) s_count[sp ^ 1] = 0;  // next pass's counter; nobody reads it before then
     const int listed = s_count[sp];
-    // Update: the waves take the listed blocks in turn (both pivots).
-    for (int u = wave; u < listed; u += W) {
+    // Update: waves 1..W-2 take the listed blocks in turn (both pivots);
+    // wave 0 stores no min-distance, so its next pass need not wait for
+    // its stores (its exchange publish among them), and the last wave is
+    // left to the speculation.
+    for (int u = wave - 1; wave >= 1 && wave <= W - 2 && u < listed; u += W - 2) {
       const uint32_t j = (uint32_t)s_list[u];
       const uint32_t base = (g + j * G) * (uint32_t)kSBlock;
       int bv = INT_MIN;
@@ -1653,7 +1657,7 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
     // only if A's block is the staged one.
     if (wave == W - 1) {
       int sj = -1;
-      if (!first && s >= f.n_starts && listed > 0 && listed < W) {
+      if (!first && s >= f.n_starts && listed > 0 && listed < 64) {
         int bv = INT_MIN;
         uint32_t bi = 0, bj = 0;
         for (uint32_t j0 = 0; j0 < nbl; j0 += 64u * 8u) {
@@ -1723,7 +1727,7 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
       if (e == 1u || listed > 0) {
         if (s_spec >= 0) {
           // A from the speculation's best of the blocks not updated (lane 0)
-          // and the updated blocks (lanes 1..listed, listed < W).
+          // and the updated blocks (lanes 1..listed, listed < 64).
           const int4 b0 = s_sb[0];
           const uint32_t jl = lane >= 1 && lane <= listed ? (uint32_t)s_list[lane - 1] : 0u;
           int v = INT_MIN;
