@@ -160,6 +160,213 @@ __global__ void __launch_bounds__(B) fps_pruned(PrunedParams a) {
 }
 
 // ---------------------------------------------------------------------------
+// One workgroup, paired at the wave level: the decision of fps_pruned_pair
+// with waves in place of workgroups and an LDS barrier in place of the
+// exchange.  Every wave keeps its best (value, original index, coordinates)
+// and its self-lookahead -- its best after a hypothetical update by its own
+// best -- the best recomputed when a pick reaches the wave's box, the
+// lookahead too (for 8 and 16 points per thread only while the wave could
+// hold the next p1: while its best beats M, the best of the other waves'
+// bests the last picks left unmoved).  After one
+// barrier every wave reads the W pairs and derives, identically: p1 the best
+// wave best; p2 the best of p1's wave's lookahead and the wave bests p1
+// leaves unmoved, unless a moved wave best could still beat it (then p1
+// alone).  One barrier per pair instead of one per pick; the picks are those
+// of the one-pick loop.  G must be 1.
+// ---------------------------------------------------------------------------
+template <int D, int B, int PPT, int TIE>
+__global__ void __launch_bounds__(B) fps_pruned_solo(PrunedParams a) {
+  constexpr int W = B / 64;
+  constexpr int PER_WG = B * PPT;
+  // Skipping lookaheads pays where a wave's sweep is long: 16 K points at 16
+  // per thread 4.55 -> 4.25 ms, against 0.74 -> 0.80 ms for 4 K points at 4,
+  // where M's reduction costs more than the sweeps it saves.
+  constexpr bool kSkip = PPT >= 8;
+  static_assert(D == 3, "the records below hold three coordinates");
+  static_assert(W <= 64, "one lane per wave in the decision");
+  // Per wave and parity: (best value, index, x, y), (z, lookahead value,
+  // index, x), (y, z, -, -).
+  __shared__ int4 s_w[2][W][3];
+  __shared__ uint32_t s_perm[PER_WG];
+
+  const FpsParams &f = a.fps;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const uint32_t wbase = (uint32_t)(wave * 64 * PPT);
+  const bool recorder = tid >= B - 64;
+
+  for (uint32_t q = tid; q < PER_WG; q += B) s_perm[q] = (q < f.n) ? a.perm[q] : 0u;
+
+  float c[PPT][D];
+  float dm[PPT];
+  float lo[D], hi[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    lo[j] = INFINITY;
+    hi[j] = -INFINITY;
+  }
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint32_t i = wbase + (uint32_t)(k * 64 + lane);
+    const bool live = i < f.n;
+    const uint32_t ic = live ? i : f.n - 1u;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const float v = a.sorted[(size_t)j * a.sorted_ld + ic];
+      c[k][j] = live ? v : 0.0f;
+      lo[j] = live ? fminf(lo[j], v) : lo[j];
+      hi[j] = live ? fmaxf(hi[j], v) : hi[j];
+    }
+    dm[k] = live ? f.init : -1.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    lo[j] = wave_min_f(lo[j]);
+    hi[j] = wave_max_f(hi[j]);
+  }
+  lds_barrier();  // s_perm complete
+
+  // Best of this wave over val(k) (value, original index, coordinates).
+  auto wave_best = [&](auto val, int &ov, int &oi, float(&oc)[D]) {
+    int bv = INT_MIN;
+    int bk = 0;
+    float bc[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) bc[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int b = __float_as_int(val(k));
+      const bool take = (TIE == kTieHighest) ? (b >= bv) : (b > bv);
+      bv = take ? b : bv;
+      bk = take ? k : bk;
+#pragma unroll
+      for (int j = 0; j < D; ++j) bc[j] = take ? c[k][j] : bc[j];
+    }
+    // Position within the block orders like the original index.
+    const uint32_t bpos = (uint32_t)(bk * 64 + lane);
+    int wv;
+    const int wl = wave_argmax_lane<TIE>(bv, bpos, wv);
+    const uint32_t wpos = (uint32_t)readlane_i((int)bpos, wl);
+#pragma unroll
+    for (int j = 0; j < D; ++j) oc[j] = readlane_f(bc[j], wl);
+    ov = wv;
+    oi = (int)s_perm[wave * 64 * PPT + wpos];
+  };
+
+  float p1[D], p2[D];
+  uint32_t i1 = load_forced<D>(f, 0, p1);
+#pragma unroll
+  for (int j = 0; j < D; ++j) p2[j] = 0.0f;
+  if (recorder) record_pick<D>(f, 0, i1, p1);
+
+  int cv = INT_MIN, ci = 0;  // this wave's best (uniform)
+  float cc[D];
+  int lv = INT_MIN, li = 0;  // and its self-lookahead
+  float lc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    cc[j] = 0.0f;
+    lc[j] = 0.0f;
+  }
+  bool cached = false;
+  bool la_valid = false;  // lv/li/lc is the lookahead of the current state
+  int mu_v = INT_MIN;     // M: best of the other waves' bests the last picks left unmoved
+  uint32_t mu_i = 0;
+  bool have2 = false;
+  uint32_t s = 1, e = 0;
+  while (s < f.m) {
+    // Apply the pending picks where they can reach.
+    const bool need1 = !cached || box_bound<D>(p1, lo, hi) < __int_as_float(cv);
+    const bool need2 = have2 && box_bound<D>(p2, lo, hi) < __int_as_float(cv);
+    if (need1) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p1), dm[k]);
+    }
+    if (need2) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) dm[k] = min_keep(sqdist<D>(c[k], p2), dm[k]);
+    }
+    if (need1 || need2) {
+      wave_best([&](int k) { return dm[k]; }, cv, ci, cc);
+      cached = true;
+      la_valid = false;
+    }
+    // The lookahead only while this wave could hold the next p1: a wave
+    // whose best does not beat M cannot (M's wave keeps its best).
+    if (!la_valid && s >= f.n_starts && (!kSkip || beats<TIE>(cv, (uint32_t)ci, mu_v, mu_i))) {
+      wave_best([&](int k) { return min_keep(sqdist<D>(c[k], cc), dm[k]); }, lv, li, lc);
+      la_valid = true;
+    }
+    if (s < f.n_starts) {
+      // Forced prefix (multi-start): one pick per pass, no decision.
+      i1 = load_forced<D>(f, s, p1);
+      have2 = false;
+      if (recorder) record_pick<D>(f, s, i1, p1);
+      ++s;
+      continue;
+    }
+    ++e;
+    const int par = (int)(e & 1u);
+    if (lane == 0) {
+      s_w[par][wave][0] = make_int4(cv, ci, __float_as_int(cc[0]), __float_as_int(cc[1]));
+      s_w[par][wave][1] = make_int4(__float_as_int(cc[2]), lv, li, __float_as_int(lc[0]));
+      s_w[par][wave][2] = make_int4(__float_as_int(lc[1]), __float_as_int(lc[2]), 0, 0);
+    }
+    lds_barrier();
+    // Every wave: one wave per lane.
+    const bool live = lane < W;
+    const int r = live ? lane : 0;
+    const int4 w0 = s_w[par][r][0], w1 = s_w[par][r][1], w2 = s_w[par][r][2];
+    const int xv = live ? w0.x : INT_MIN;
+    const uint32_t xi = (uint32_t)w0.y;
+    int v1;
+    const int l1 = wave_argmax_lane<TIE>(xv, xi, v1);
+    i1 = (uint32_t)readlane_i((int)xi, l1);
+    const float q1[D] = {__int_as_float(readlane_i(w0.z, l1)), __int_as_float(readlane_i(w0.w, l1)),
+                         __int_as_float(readlane_i(w1.x, l1))};
+    const float qa[D] = {__int_as_float(w0.z), __int_as_float(w0.w), __int_as_float(w1.x)};
+    // Each wave best after p1, in the update's own arithmetic.
+    const float post = min_keep(sqdist<D>(qa, q1), __int_as_float(xv));
+    const bool moved = live && lane != l1 && __float_as_int(post) != xv;
+    const bool mine = lane == l1;
+    const int tv = mine ? w1.y : ((live && !moved) ? xv : INT_MIN);
+    const uint32_t ti = (uint32_t)(mine ? w1.z : w0.y);
+    int v2;
+    const int l2 = wave_argmax_lane<TIE>(tv, ti, v2);
+    const uint32_t i2 = (uint32_t)readlane_i((int)ti, l2);
+    const bool beaten = moved && !beats<TIE>(v2, i2, xv, xi);
+    const bool two = __ballot(beaten) == 0;
+    const float q2[D] = {__int_as_float(readlane_i(mine ? w1.w : w0.z, l2)),
+                         __int_as_float(readlane_i(mine ? w2.x : w0.w, l2)),
+                         __int_as_float(readlane_i(mine ? w2.y : w1.x, l2))};
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      p1[j] = q1[j];
+      p2[j] = q2[j];
+    }
+    if constexpr (kSkip) {
+      // M for the next round, this wave's own: the best of the other waves'
+      // bests that keep their value under the picks (so stay their bests).
+      const float post2 = min_keep(sqdist<D>(qa, q2), __int_as_float(xv));
+      const bool keep = live && __float_as_int(post) == xv && !(two && __float_as_int(post2) != xv);
+      int mv;
+      const int ml = wave_argmax_lane<TIE>((keep && lane != wave) ? xv : INT_MIN, xi, mv);
+      mu_v = mv;
+      mu_i = (uint32_t)readlane_i((int)xi, ml);
+    }
+    have2 = two && s + 1 < f.m;
+    if (recorder) {
+      record_pick<D>(f, s, i1, p1);
+      if (have2) record_pick<D>(f, s + 1, i2, p2);
+    }
+    s += have2 ? 2u : 1u;
+  }
+  // The launch's round count in the second status word.
+  if (recorder && lane == 0) f.status[1] = e;
+}
+
+// ---------------------------------------------------------------------------
 // Paired form: one cross-workgroup exchange settles up to two picks.
 //
 // Every workgroup publishes two candidates: its best point A (as above) and
@@ -2132,6 +2339,20 @@ static const KernelEntry kPruned[] = {
 const KernelEntry *pruned_table(size_t *count) {
   *count = sizeof(kPruned) / sizeof(kPruned[0]);
   return kPruned;
+}
+
+#define FPSX_SOLO(D, B, P)                                                        \
+  {D, B, P, kTieHighest, (const void *)&fps_pruned_solo<D, B, P, kTieHighest>}, \
+  {D, B, P, kTieLowest, (const void *)&fps_pruned_solo<D, B, P, kTieLowest>}
+
+static const KernelEntry kSolo[] = {
+    FPSX_SOLO(3, 1024, 16), FPSX_SOLO(3, 1024, 8), FPSX_SOLO(3, 1024, 4),
+};
+#undef FPSX_SOLO
+
+const KernelEntry *solo_table(size_t *count) {
+  *count = sizeof(kSolo) / sizeof(kSolo[0]);
+  return kSolo;
 }
 
 // (points per thread P = registers PPT + LDS E)

@@ -17,6 +17,9 @@ struct PrunedParams {
 };
 
 const KernelEntry *pruned_table(size_t *count);
+// One workgroup paired at the wave level (same parameters, G = 1): up to
+// two picks per LDS barrier.
+const KernelEntry *solo_table(size_t *count);
 // Paired form (same parameters): up to two picks per exchange; 2 <= G <= 64.
 const KernelEntry *pair_table(size_t *count);
 constexpr uint32_t kPairMaxGrid = 64;

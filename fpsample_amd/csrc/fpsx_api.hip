@@ -151,6 +151,14 @@ static const KernelEntry *find_pruned(int dim, int B, int ppt, int tie) {
   return nullptr;
 }
 
+static const KernelEntry *find_solo(int dim, int B, int ppt, int tie) {
+  size_t cnt;
+  const KernelEntry *t = solo_table(&cnt);
+  for (size_t i = 0; i < cnt; ++i)
+    if (t[i].dim == dim && t[i].block == B && t[i].ppt == ppt && t[i].tie == tie) return &t[i];
+  return nullptr;
+}
+
 static const KernelEntry *find_pair(int dim, int B, int ppt, int tie) {
   size_t cnt;
   const KernelEntry *t = pair_table(&cnt);
@@ -284,6 +292,15 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
         return fail(FPSX_ETOOBIG, "FPSX_PLAN: the paired kernel needs 2..256 workgroups: " + s);
       return FPSX_OK;
     }
+    if (std::sscanf(s.c_str(), "solo:%d:%d", &a, &b) == 2) {
+      // One workgroup paired at the wave level.
+      const KernelEntry *k = find_solo((int)dim, a, b, tie);
+      if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no one-workgroup paired kernel for dim " +
+                                           std::to_string(dim) + " shape " + s);
+      if (plan_resident_shape(c, n, k, p, PlanKind::Pair) != FPSX_OK || p.grid != 1)
+        return fail(FPSX_ETOOBIG, "FPSX_PLAN: the cloud does not fit one workgroup of " + s);
+      return FPSX_OK;
+    }
     if (std::sscanf(s.c_str(), "pruned:%d:%d", &a, &b) == 2) {
       const KernelEntry *k = find_pruned((int)dim, a, b, tie);
       if (!k) return fail(FPSX_EINVAL, "FPSX_PLAN: no pruned kernel for dim " +
@@ -324,10 +341,14 @@ int plan(Ctx &c, size_t n, size_t dim, int tie, size_t m, Plan &p) {
   const char *npair = std::getenv("FPSX_NO_PAIR");
   const bool prune = dim == 3 && !(np && np[0] == '1');
   if (prune && n <= 16384 && ((n >= 4096 && m >= 1024) || (n > 4096 && m >= 512))) {
+    // One workgroup, paired at the wave level (up to two picks per barrier)
+    // unless FPSX_NO_PAIR=1.
+    const bool solo = !(npair && npair[0] == '1');
     for (int ppt : {4, 8, 16}) {
       if ((size_t)1024 * ppt < n) continue;
-      const KernelEntry *k = find_pruned(3, 1024, ppt, tie);
-      if (k && plan_resident_shape(c, n, k, p, PlanKind::Pruned) == FPSX_OK) return FPSX_OK;
+      const KernelEntry *k = solo ? find_solo(3, 1024, ppt, tie) : find_pruned(3, 1024, ppt, tie);
+      if (k && plan_resident_shape(c, n, k, p, solo ? PlanKind::Pair : PlanKind::Pruned) == FPSX_OK)
+        return FPSX_OK;
     }
   }
   if (prune && n > 16384 && m >= 1024) {
@@ -555,6 +576,11 @@ static std::vector<Plan> candidates(Ctx &c, size_t n, size_t dim, int tie, size_
       if (t[i].dim == 3 && t[i].tie == tie &&
           plan_resident_shape(c, n, &t[i], q, PlanKind::Pair) == FPSX_OK && q.grid >= 2 &&
           q.grid <= kPairMaxGrid)
+        out.push_back(q);
+    t = solo_table(&cnt);
+    for (size_t i = 0; i < cnt; ++i)
+      if (t[i].dim == 3 && t[i].tie == tie &&
+          plan_resident_shape(c, n, &t[i], q, PlanKind::Pair) == FPSX_OK && q.grid == 1)
         out.push_back(q);
     for (int pw : {2, 4}) {
       const KernelEntry *k = pair_wide_kernel(1024, 16, tie, pw);
@@ -958,7 +984,8 @@ FPSX_EXPORT int fpsx_launch_exchanges(void *stream, uint64_t *exchanges) {
   Ctx &c = *cp;
   std::lock_guard<std::mutex> lk(c.mu);
   *exchanges = 0;
-  if (!c.last_plan.k || c.last_plan.grid < 2) return FPSX_OK;
+  if (!c.last_plan.k) return FPSX_OK;
+  if (c.last_plan.grid < 2 && c.last_plan.kind != PlanKind::Pair) return FPSX_OK;
   if (c.last_plan.kind != PlanKind::Pair && c.last_plan.kind != PlanKind::StreamPair) {
     // One exchange per pick after the forced prefix.
     *exchanges = c.last_m > c.last_starts ? c.last_m - c.last_starts : 0;

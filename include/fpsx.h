@@ -78,12 +78,14 @@ void *fpsx_stream(void);
  * 5 = kind 2 settling up to two picks per cross-workgroup exchange (2..256
  * workgroups; ppt 17 = 1024 x 16 points in registers plus one per thread in
  * LDS, taken for 2^20 < n_points <= 1,114,112; beyond 64 workgroups, 1024 x 16
- * with 2 or 4 waves polling the exchange), 6 = kind 4 settling up to two
- * picks per exchange (2..64 workgroups);
+ * with 2 or 4 waves polling the exchange), or with grid 1 up to two picks per
+ * workgroup barrier (the waves paired instead of workgroups), 6 = kind 4
+ * settling up to two picks per exchange (2..64 workgroups);
  * threads per workgroup, points per thread (0 when streaming), workgroups,
  * points per workgroup.  FPSX_PLAN in the environment forces a plan:
  * "resident:B:PPT", "pruned:B:PPT", "pair:B:PPT", "pairwPW:B:PPT" (the paired
- * kernel with PW = 2 or 4 polling waves at any grid it holds), "stream:B[:G]",
+ * kernel with PW = 2 or 4 polling waves at any grid it holds), "solo:B:PPT" (one
+ * workgroup paired at the wave level), "stream:B[:G]",
  * "generic:B[:G]", "spruned[:G]", "spair[:G]";
  * FPSX_NO_PRUNE=1 keeps the planner off the pruned kinds, FPSX_NO_PAIR=1 off
  * kinds 5 and 6.
@@ -119,7 +121,8 @@ int fpsx_exchange_timeout(double seconds);
 /*
  * Number of cross-workgroup exchanges the last FPS launch on the current
  * device made (after `stream`, NULL: the library's, reaches it): 0 for a
- * one-workgroup plan, one per pick after the forced prefix for the one-pick
+ * one-workgroup plan (but for kind 5 with one workgroup: its rounds, one
+ * barrier each), one per pick after the forced prefix for the one-pick
  * kernels, counted by the kernel for the paired kinds (up to two picks per
  * exchange; kind 5 also exchanges workgroup boxes once).  Measurement aid; no
  * counterpart in the reference.

@@ -274,14 +274,15 @@ def test_pruned_auto_plan_and_bucket_order(gpu):
 
 
 def test_small_clouds_auto_plans(gpu):
-    # The planner's choices below 64 K points: one pruned workgroup (kind 2,
-    # grid 1, the smallest share holding the cloud) from 4 K points and 1024
-    # samples or 8 K points and 512 samples; the paired kernel at 4 points per
-    # thread above 16 K points; the resident kernel otherwise.  Every choice
-    # against the oracle, both tie rules (the bucket entry ties lowest).
+    # The planner's choices below 64 K points: one workgroup paired at the
+    # wave level (kind 5, grid 1, the smallest share holding the cloud) from
+    # 4 K points and 1024 samples or 8 K points and 512 samples; the paired
+    # kernel at 4 points per thread above 16 K points; the resident kernel
+    # otherwise.  Every choice against the oracle, both tie rules (the bucket
+    # entry ties lowest).
     rng = np.random.default_rng(35)
-    cases = [(4096, 1024, 2, 4), (5000, 600, 2, 8), (8192, 2048, 2, 8), (12000, 3000, 2, 16),
-             (16384, 4096, 2, 16), (16385, 1024, 5, 4), (40000, 2500, 5, 4),
+    cases = [(4096, 1024, 5, 4), (5000, 600, 5, 8), (8192, 2048, 5, 8), (12000, 3000, 5, 16),
+             (16384, 4096, 5, 16), (16385, 1024, 5, 4), (40000, 2500, 5, 4),
              (4096, 256, 0, None), (3000, 1500, 0, None), (30000, 500, 0, None)]
     for n, m, kind, ppt in cases:
         info = gpu.plan_info(n, 3, m=m)
@@ -439,3 +440,56 @@ def test_fused_gather_pruned_kinds(gpu, plan_env, plan):
     D.device_status()
     np.testing.assert_array_equal(out.numpy(), want, err_msg=plan)
     np.testing.assert_array_equal(opts.numpy(), pc[want.astype(np.int64)], err_msg=plan)
+
+
+SOLO_PLANS = ["solo:1024:4", "solo:1024:8", "solo:1024:16", "NO_PAIR"]
+
+
+@pytest.mark.parametrize("plan", SOLO_PLANS)
+def test_one_workgroup_paired_plans(gpu, plan_env, monkeypatch, plan):
+    # One workgroup paired at the wave level (and the one-pick workgroup
+    # under FPSX_NO_PAIR=1): uniform, clustered, heavy duplicates (every
+    # point picked, so the last rounds pick among zero min-distances), special
+    # values; forced prefix, both tie rules, a large initial distance.
+    from fpsample_amd import device as D
+
+    if plan == "NO_PAIR":
+        plan_env(None)
+        monkeypatch.setenv("FPSX_NO_PAIR", "1")
+    else:
+        plan_env(plan)
+    rng = np.random.default_rng(44)
+    n = 4096
+    dups = np.repeat(rng.integers(0, 5, (n // 4, 3)), 4, axis=0).astype(np.float32)
+    special = (rng.random((n, 3)) * 4e19).astype(np.float32)
+    special[[5, 700, 4000]] = np.nan
+    special[9] = np.inf
+    clouds = [("uniform", rng.random((n, 3)).astype(np.float32), 1024),
+              ("gauss", c2_cloud(n, seed=12), 2048),
+              ("dups_all", dups, n),
+              ("special", special, 1500)]
+    for name, pc, m in clouds:
+        info = gpu.plan_info(len(pc), 3, m=m)
+        log(f"solo {plan} {name} m={m} plan={info}")
+        assert info["grid"] == 1, info
+        np.testing.assert_array_equal(run_fps(gpu, pc, m, 7), O.fps(pc, m, 7), err_msg=name)
+        np.testing.assert_array_equal(run_fps(gpu, pc, m, [3, 3, 100]), O.fps(pc, m, [3, 3, 100]),
+                                      err_msg=name + " starts")
+        soa = D.DeviceArray.from_numpy(np.ascontiguousarray(pc.T))
+        ds = D.DeviceArray.from_numpy(np.array([11], np.uint64))
+        out = D.DeviceArray((m,), np.uint64)
+        D.fps_sampling_device(soa, len(pc), 3, m, ds, 1, out, tie=D.TIE_LOWEST,
+                              init=float(np.finfo(np.float32).max))
+        D.synchronize()
+        D.device_status()
+        want = O.fps_ex(pc, m, 11, lowest=True, init=float(np.finfo(np.float32).max))
+        np.testing.assert_array_equal(out.numpy(), want, err_msg=name + " lowest")
+        # Both picks of a round write their coordinates (fused gather).
+        opts = D.DeviceArray((m, 3), np.float32)
+        ds2 = D.DeviceArray.from_numpy(np.array([17, 3], np.uint64))
+        D.fps_sampling_device_strided(soa, 1, len(pc), len(pc), 3, m, ds2, 2, out, opts)
+        D.synchronize()
+        D.device_status()
+        want = O.fps(pc, m, [17, 3])
+        np.testing.assert_array_equal(out.numpy(), want, err_msg=name + " gather")
+        np.testing.assert_array_equal(opts.numpy(), pc[want.astype(np.int64)], err_msg=name + " gather")
