@@ -493,3 +493,10 @@ def test_one_workgroup_paired_plans(gpu, plan_env, monkeypatch, plan):
         want = O.fps(pc, m, [17, 3])
         np.testing.assert_array_equal(out.numpy(), want, err_msg=name + " gather")
         np.testing.assert_array_equal(opts.numpy(), pc[want.astype(np.int64)], err_msg=name + " gather")
+    if plan != "NO_PAIR":
+        # Clouds far below a workgroup's share (empty waves), one and two picks.
+        for n_small, m_small in ((37, 37), (1000, 700), (5, 1), (64, 2)):
+            pc = rng.random((n_small, 3)).astype(np.float32)
+            log(f"solo {plan} n={n_small} m={m_small}")
+            np.testing.assert_array_equal(run_fps(gpu, pc, m_small, 0), O.fps(pc, m_small, 0),
+                                          err_msg=f"n={n_small} m={m_small}")
