@@ -396,6 +396,20 @@ def test_lattice_streaming_kernels_agree(gpu, plan_env):
     np.testing.assert_array_equal(a[:24], O.fps(pc, 24, 0))
 
 
+@pytest.mark.parametrize("plan", ["spair:3", "spair:8"])
+def test_stream_pair_large_share(gpu, plan_env, plan):
+    # Over 960 blocks per workgroup at 3 workgroups (the box tests reach the
+    # last wave, which also speculates; several steps of every block scan)
+    # and 512 at 8; clustered and lattice clouds of 2^21 points.
+    plan_env(plan)
+    rng = np.random.default_rng(45)
+    n = 1 << 21
+    for name, pc in (("gauss", c2_cloud(n, seed=21)),
+                     ("lattice", rng.integers(0, 128, (n, 3)).astype(np.float32))):
+        log(f"spair large share {plan} {name} plan={gpu.plan_info(n, 3, m=1200)}")
+        np.testing.assert_array_equal(run_fps(gpu, pc, 1200, 5), O.fps(pc, 1200, 5), err_msg=name)
+
+
 @pytest.mark.parametrize("plan", ["spair:2", "spair:7"])
 def test_stream_pair_exhausts_cloud(gpu, plan_env, plan):
     # Every point picked from a cloud of duplicates: the later rounds pick
