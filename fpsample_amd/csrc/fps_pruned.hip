@@ -1663,16 +1663,21 @@ __global__ void __launch_bounds__(1024) fps_stream_pruned(PrunedParams a) {
 // settling up to two picks per exchange as fps_pruned_pair does.  Every
 // workgroup publishes its best A and a lookahead A' (its best after a
 // hypothetical update by A); the polled pairs give p1 and, unless some
-// workgroup whose A moves under p1 could still beat it, p2.  For the
-// lookahead every wave tests its share of the blocks: a block whose box lies
-// out of A's reach, or whose cached best keeps its value under A, keeps its
-// best; the others are listed and streamed, one per wave (their min-distances
-// min'ed again with this pass's pivots -- a no-op once this pass's stores
-// have landed -- and with A; nothing written back), and wave 0 takes the best
-// of the waves' results.  A workgroup skips the lookahead while
-// the best A of the workgroups the last picks certainly left unchanged (their
-// A keeps its value, so it stays their best) beats its own: it cannot then
-// hold the next p1, the only workgroup whose A' is read.  2 <= G <= 64.
+// workgroup whose A moves under p1 could still beat it, p2.  While waves
+// 1..14 update the listed blocks, the idle last wave speculates on A (the
+// best cached candidate of the blocks not updated) and stages that block in
+// LDS; A is then formed from its candidate and the updated blocks.  For the
+// lookahead, waves 0..7 test the blocks: a block whose box lies out of A's
+// reach, or whose cached best keeps its value under A, keeps its best; the
+// others are listed and streamed (their min-distances min'ed again with this
+// pass's pivots where written this pass -- a no-op once the stores have
+// landed -- and with A; nothing written back).  Waves 8..15 meanwhile take
+// A's own block one row each, from the staged copy when the guess held.
+// Wave 0 takes the best of the waves' results.  A workgroup skips the
+// lookahead while the best A of the workgroups the last picks certainly left
+// unchanged (their A keeps its value, so it stays their best) beats its own:
+// it cannot then hold the next p1, the only workgroup whose A' is read.
+// 2 <= G <= 64.
 // ---------------------------------------------------------------------------
 template <int TIE>
 __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
