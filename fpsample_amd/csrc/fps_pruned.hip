@@ -1688,7 +1688,7 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
   constexpr int NP = 2 * NW;
   constexpr int SL = kSBlock / 64;  // slots per lane in a block
   __shared__ int4 s_res[2][3];      // p1, p2, two-pick flag, all-arrived flag
-  extern __shared__ int s_meta[];   // kSPairMetaWords rows of nbl words
+  extern __shared__ __align__(16) int s_meta[];  // kSPairMetaWords words per block
   __shared__ int s_count[2];        // listed blocks, by pass parity
   __shared__ int4 s_a[2];           // A (value, index, coordinates), lookahead flag
   __shared__ int s_cnt2;            // blocks the lookahead streams
@@ -1709,12 +1709,16 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
   const uint32_t n = f.n;
   const uint32_t nblocks = (n + kSBlock - 1) / kSBlock;
   const uint32_t nbl = (nblocks > g) ? (nblocks - g + G - 1) / G : 0u;
-  float *m_lo = reinterpret_cast<float *>(s_meta);  // m_lo[c * nbl + j]
-  float *m_hi = m_lo + 3 * nbl;
-  int *m_v = reinterpret_cast<int *>(m_hi + 3 * nbl);
+  // Per block j, 16-byte records (read with one LDS instruction each): box
+  // low corner and cached best value bits, box high corner and cached best
+  // index, cached best coordinates; the value and index again by themselves
+  // for the scans that need only them.
+  float4 *m_b0 = reinterpret_cast<float4 *>(s_meta);  // lo.x, lo.y, lo.z, value bits
+  float4 *m_b1 = m_b0 + nbl;                          // hi.x, hi.y, hi.z, index bits
+  float4 *m_cb = m_b1 + nbl;                          // best x, y, z, -
+  int *m_v = reinterpret_cast<int *>(m_cb + nbl);
   int *m_i = m_v + nbl;
-  int *m_c = m_i + nbl;  // m_c[c * nbl + j]: coordinate bits of the block's best
-  int *s_list = m_c + 3 * nbl;  // blocks to stream (update, then the lookahead)
+  int *s_list = m_i + nbl;  // blocks to stream (update, then the lookahead)
   uint32_t *m_mark = reinterpret_cast<uint32_t *>(s_list + nbl);  // pass that last listed the slot
   for (uint32_t j = (uint32_t)tid; j < nbl; j += B) m_mark[j] = 0u;
   if (tid < 2) s_count[tid] = 0;
@@ -1766,9 +1770,10 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
         if (first) {
           need = true;
         } else {
-          const float lo[D] = {m_lo[jl], m_lo[nbl + jl], m_lo[2 * nbl + jl]};
-          const float hi[D] = {m_hi[jl], m_hi[nbl + jl], m_hi[2 * nbl + jl]};
-          const float bvf = __int_as_float(m_v[jl]);
+          const float4 b0 = m_b0[jl], b1 = m_b1[jl];
+          const float lo[D] = {b0.x, b0.y, b0.z};
+          const float hi[D] = {b1.x, b1.y, b1.z};
+          const float bvf = b0.w;
           need = box_bound<D>(p1, lo, hi) < bvf || (have2 && box_bound<D>(p2, lo, hi) < bvf);
         }
       }
@@ -1849,14 +1854,13 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
       if (lane == 0) {
         m_v[j] = wv;
         m_i[j] = wi;
-#pragma unroll
-        for (int q = 0; q < D; ++q) m_c[q * nbl + j] = __float_as_int(wc[q]);
+        m_cb[j] = make_float4(wc[0], wc[1], wc[2], 0.0f);
         if (first) {
-#pragma unroll
-          for (int q = 0; q < D; ++q) {
-            m_lo[q * nbl + j] = lo[q];
-            m_hi[q * nbl + j] = hi[q];
-          }
+          m_b0[j] = make_float4(lo[0], lo[1], lo[2], __int_as_float(wv));
+          m_b1[j] = make_float4(hi[0], hi[1], hi[2], __int_as_float(wi));
+        } else {
+          m_b0[j].w = __int_as_float(wv);
+          m_b1[j].w = __int_as_float(wi);
         }
       }
     }
@@ -1901,8 +1905,9 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
           if (lane == 0) {
             // The best cached candidate of the blocks not updated this pass:
             // wave 0 then forms A from it and the updated blocks alone.
-            s_sb[0] = make_int4(gv, readlane_i((int)bi, gl), (int)tj, m_c[tj]);
-            s_sb[1] = make_int4(m_c[nbl + tj], m_c[2 * nbl + tj], 0, 0);
+            const float4 cb = m_cb[tj];
+            s_sb[0] = make_int4(gv, readlane_i((int)bi, gl), (int)tj, __float_as_int(cb.x));
+            s_sb[1] = make_int4(__float_as_int(cb.y), __float_as_int(cb.z), 0, 0);
           }
           const uint32_t base = (g + tj * G) * (uint32_t)kSBlock;
           float4 r[SL];
@@ -1964,8 +1969,10 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
             ga[3] = b1.x;
             ga[4] = b1.y;
           } else {
-#pragma unroll
-            for (int q = 0; q < D; ++q) ga[2 + q] = m_c[q * nbl + wj];
+            const float4 cb = m_cb[wj];
+            ga[2] = __float_as_int(cb.x);
+            ga[3] = __float_as_int(cb.y);
+            ga[4] = __float_as_int(cb.z);
           }
           ga_j = wj;
           la_valid = false;
@@ -1999,8 +2006,10 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
           const uint32_t wj = (uint32_t)readlane_i((int)bj, gl);
           ga[0] = gv;
           ga[1] = readlane_i((int)bi, gl);
-#pragma unroll
-          for (int q = 0; q < D; ++q) ga[2 + q] = m_c[q * nbl + wj];
+          const float4 cb = m_cb[wj];
+          ga[2] = __float_as_int(cb.x);
+          ga[3] = __float_as_int(cb.y);
+          ga[4] = __float_as_int(cb.z);
           ga_j = wj;
           la_valid = false;
         }
@@ -2096,11 +2105,12 @@ __global__ void __launch_bounds__(1024) fps_stream_pair(PrunedParams a) {
           const uint32_t jl = j0 + (uint32_t)lane;
           bool sweep = false;
           if (jl < nbl) {
-            const int v = m_v[jl];
-            const uint32_t id = (uint32_t)m_i[jl];
-            const float lo[D] = {m_lo[jl], m_lo[nbl + jl], m_lo[2 * nbl + jl]};
-            const float hi[D] = {m_hi[jl], m_hi[nbl + jl], m_hi[2 * nbl + jl]};
-            const int cb[D] = {m_c[jl], m_c[nbl + jl], m_c[2 * nbl + jl]};
+            const float4 b0 = m_b0[jl], b1 = m_b1[jl], c4 = m_cb[jl];
+            const int v = __float_as_int(b0.w);
+            const uint32_t id = (uint32_t)__float_as_int(b1.w);
+            const float lo[D] = {b0.x, b0.y, b0.z};
+            const float hi[D] = {b1.x, b1.y, b1.z};
+            const int cb[D] = {__float_as_int(c4.x), __float_as_int(c4.y), __float_as_int(c4.z)};
             const float cf[D] = {__int_as_float(cb[0]), __int_as_float(cb[1]), __int_as_float(cb[2])};
             const float vf = __int_as_float(v);
             const bool kept = !(box_bound<D>(pa, lo, hi) < vf) ||

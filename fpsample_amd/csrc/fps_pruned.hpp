@@ -35,9 +35,11 @@ constexpr uint32_t kPairWideMaxGrid = 256;
 constexpr int kSBlock = 512;
 constexpr int kSMetaWords = 12;
 const KernelEntry *stream_pruned_table(size_t *count);
-// Paired streaming form (same parameters; one more LDS word per block, the
-// pass that last listed it): up to two picks per exchange; 2 <= G <= kPairMaxGrid.
-constexpr int kSPairMetaWords = kSMetaWords + 1;
+// Paired streaming form (same parameters; 16 LDS words per block: 16-byte
+// records of the box and cached candidate, the candidate's value and index
+// again, a list entry and the pass that last listed it): up to two picks per
+// exchange; 2 <= G <= kPairMaxGrid.
+constexpr int kSPairMetaWords = 16;
 const KernelEntry *stream_pair_table(size_t *count);
 
 // Device scratch of the spatial ordering (grow-only, owned by the caller).

@@ -219,7 +219,8 @@ static int plan_stream_pruned(Ctx &c, size_t n, int tie, size_t want_g, Plan &p,
     return fail(FPSX_ETOOBIG, "paired streaming plan: needs 2..64 workgroups");
   size_t nbl = (nblocks + g - 1) / g;
   size_t shmem = (size_t)(pair ? kSPairMetaWords : kSMetaWords) * nbl * sizeof(int);
-  if (shmem > 104 * 1024) return fail(FPSX_ETOOBIG, "streaming pruned plan: too many blocks per workgroup");
+  if (shmem > (pair ? 128 : 96) * 1024)
+    return fail(FPSX_ETOOBIG, "streaming pruned plan: too many blocks per workgroup");
   const size_t gmax = (size_t)c.max_blocks(k, shmem);
   if (g > gmax) return fail(FPSX_ETOOBIG, "streaming pruned plan: grid exceeds co-residency");
   p.kind = pair ? PlanKind::StreamPair : PlanKind::StreamPruned;
