@@ -127,8 +127,20 @@ WORKLOADS = {
     "gauss2m": dict(gen=lambda: gen_gauss_mixture(1 << 21), m=16384,
                     desc="one cloud N=2097152 D=3 f32, 64 Gaussian clusters sigma 0.02, "
                          "M=16384, start_idx=0"),
+    # the reference's own vanilla bench cases (bench/test_bench.py:29-33, 89-93, 143-147;
+    # its README publishes their mean times on a Xeon Gold 6326, one thread, with a
+    # random start where these use start_idx 0)
     "uniform4k": dict(gen=gen_uniform_4096, m=1024,
-                      desc="one cloud N=4096 D=3 f32 uniform [0,1)^3 seed 42, M=1024, start_idx=0"),
+                      desc="one cloud N=4096 D=3 f32 uniform [0,1)^3 seed 42, M=1024, start_idx=0",
+                      published=(12.5875e-3, "README.md:123")),
+    "uniform50k": dict(gen=lambda: gen_uniform_n(50_000), m=4096,
+                       desc="one cloud N=50000 D=3 f32 uniform [0,1)^3 seed 42, M=4096, "
+                            "start_idx=0",
+                       published=(593.2049e-3, "README.md:136")),
+    "uniform100k": dict(gen=lambda: gen_uniform_n(100_000), m=50_000,
+                        desc="one cloud N=100000 D=3 f32 uniform [0,1)^3 seed 42, M=50000, "
+                             "start_idx=0",
+                        published=(14561.1574e-3, "README.md:148")),
     "lattice16m": dict(gen=gen_lattice_dups, m=4096,
                        desc="one cloud N=16777216 D=3 f32 lattice {0..255}^3, 25% duplicates, "
                             "M=4096, start_idx=0"),
@@ -813,16 +825,21 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": None,
     }
-    if not npdu and args.workload in ("gauss1m", "gauss1m1", "gauss2m", "lattice16m"):
-        ref = None
+    if not npdu:
         try:
             from tests.golden import full_length as FL
 
             if FL.available(args.workload):
-                ref = bool(np.array_equal(out, FL.load(args.workload)["out"]))
+                result["identical_to_reference"] = bool(
+                    np.array_equal(out, FL.load(args.workload)["out"]))
         except ImportError:
             pass
-        result["identical_to_reference"] = ref
+    if "published" in wl:
+        mean_s, where = wl["published"]
+        result["reference_published"] = {
+            "value": round(m / mean_s, 3), "unit": "samples/s", "cores": 1,
+            "hardware": "Xeon Gold 6326 @ 2.90 GHz", "source": where,
+            "gpu_over_published": round(value / (m / mean_s), 2)}
     if args.cpu_baseline == "on":
         result["cpu_baseline"] = cpu_baseline(pc, m, args.cpu_seconds, w)
         if args.workload == "gauss1m":

@@ -11,6 +11,9 @@ the cloud itself:
   gauss1m1  the same generator at N = 2^20 + 1
   gauss2m   the same generator at N = 2^21
   lattice16m  N = 2^24 lattice {0..255}^3 with 25 % duplicates (default_rng(4)), M = 4096
+  uniform50k  N = 50000 uniform in [0,1)^3 (np.random.seed(42)), M = 4096, start 0:
+            the reference's own 50k bench case (bench/test_bench.py:89-93)
+  uniform100k N = 100000, same generator, M = 50000, start 0 (bench/test_bench.py:143-147)
   batch256  the 256 ragged clouds (default_rng(2)), M_i = N_i // 4: per cloud the
             SHA-256 of its picks (uint64 little-endian) and its first 16 picks
   serve96   the 96 serving-stream requests (default_rng(3)), M = N * U{1/8,1/4,1/2}:
@@ -58,7 +61,12 @@ def _single(task):
     name, n, m = task
     import bench
 
-    pc = bench.gen_lattice_dups(n) if name.startswith("lattice") else bench.gen_gauss_mixture(n)
+    if name.startswith("lattice"):
+        pc = bench.gen_lattice_dups(n)
+    elif name.startswith("uniform"):
+        pc = bench.gen_uniform_n(n)
+    else:
+        pc = bench.gen_gauss_mixture(n)
     t0 = time.time()
     out = np.asarray(_ref().fps_sampling(pc, m, 0), np.uint64)
     assert out.max() < 2**32
@@ -90,7 +98,8 @@ def _many(name, clouds, ms, pool):
 
 
 SINGLES = {"lattice16m": (1 << 24, 4096), "gauss2m": (1 << 21, 16384),
-           "gauss1m": (1 << 20, 16384), "gauss1m1": ((1 << 20) + 1, 16384)}
+           "gauss1m": (1 << 20, 16384), "gauss1m1": ((1 << 20) + 1, 16384),
+           "uniform100k": (100_000, 50_000), "uniform50k": (50_000, 4096)}
 
 
 def main(names):

@@ -7,6 +7,8 @@ hold every pick of
   - the 1 M-point Gaussian mixture, 16,384 picks, and the same generator at
     2^20 + 1 and 2^21 points;
   - the 16.7 M-point lattice with 25 % duplicates, 4,096 picks (heavy ties);
+  - the reference's own 50k and 100k bench cases (uniform, np.random.seed(42);
+    4,096 of 50,000 and 50,000 of 100,000 picks);
 and a SHA-256 per cloud of every pick of
   - the 256 ragged clouds of the batch workload (M_i = N_i / 4);
   - the 96 requests of the serving workload (N up to 2^20, M = N/8..N/2).
@@ -85,6 +87,17 @@ def lattice16m():
                          ids=["auto", "spruned", "stream1024", "spruned128"])
 def test_lattice16m_every_pick(gpu, monkeypatch, lattice16m, env):
     check_single(gpu, monkeypatch, "lattice16m", lattice16m, env)
+
+
+@pytest.mark.parametrize("n,name,env", [(50_000, "uniform50k", {}),
+                                        (50_000, "uniform50k", {"FPSX_NO_PAIR": "1"}),
+                                        (100_000, "uniform100k", {}),
+                                        (100_000, "uniform100k", {"FPSX_NO_PAIR": "1"}),
+                                        (100_000, "uniform100k", {"FPSX_NO_PRUNE": "1"})],
+                         ids=["50k-auto", "50k-no_pair", "100k-auto", "100k-no_pair",
+                              "100k-no_prune"])
+def test_reference_bench_cases_every_pick(gpu, monkeypatch, n, name, env):
+    check_single(gpu, monkeypatch, name, bench.gen_uniform_n(n), env)
 
 
 def check_many(name, clouds, ms, got):

@@ -100,6 +100,18 @@ def test_brute_force_tiny(seed):
     assert O.fps_ex(pc, m, starts, lowest=True).tolist() == _brute_force(pc, m, starts, True)
 
 
+def test_full_length_uniform50k():
+    # Every pick of the reference's own 50k bench case (bench/test_bench.py:89-93,
+    # N=50000, M=4096, start 0), written by the compiled reference.
+    import bench
+    from tests.golden import full_length as FL
+
+    d = FL.load("uniform50k")
+    pc = bench.gen_uniform_n(50_000)
+    assert FL.sha256(pc) == d["cloud_sha256"]
+    np.testing.assert_array_equal(O.fps(pc, int(d["m"]), 0), d["out"])
+
+
 def test_numpy_and_c_agree_on_ties():
     rng = np.random.default_rng(7)
     pc = rng.integers(0, 4, (400, 3)).astype(np.float32)
