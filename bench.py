@@ -144,6 +144,17 @@ WORKLOADS = {
     "lattice16m": dict(gen=gen_lattice_dups, m=4096,
                        desc="one cloud N=16777216 D=3 f32 lattice {0..255}^3, 25% duplicates, "
                             "M=4096, start_idx=0"),
+    # bucket_fps_kdline_sampling (h = 7) on the same two clouds (bench/test_bench.py:131-135,
+    # 178-182): host arrays in and out, so each step includes the host KD-line build and
+    # the PCIe copies; the README publishes the reference's mean times
+    "kdline50k": dict(gen=lambda: gen_uniform_n(50_000), m=4096, kind="kdline", h=7,
+                      desc="bucket_fps_kdline_sampling h=7 on host arrays, one cloud N=50000 "
+                           "D=3 f32 uniform [0,1)^3 seed 42, M=4096, start_idx=0",
+                      published=(25.0179e-3, "README.md:130")),
+    "kdline100k": dict(gen=lambda: gen_uniform_n(100_000), m=50_000, kind="kdline", h=7,
+                       desc="bucket_fps_kdline_sampling h=7 on host arrays, one cloud N=100000 "
+                            "D=3 f32 uniform [0,1)^3 seed 42, M=50000, start_idx=0",
+                       published=(233.3726e-3, "README.md:142")),
     # windowed-update FPS on the reference's own bench cases (bench/test_bench.py:43-46,
     # 151-154): w is the Python wrapper's default int(N / M * 16)
     "npdu4k": dict(gen=gen_uniform_4096, m=1024, kind="npdu",
@@ -184,8 +195,10 @@ class _pinned:
             os.sched_setaffinity(0, self.saved)
 
 
-def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float, w: int = -1) -> dict:
-    """w < 0: fps_sampling; else fps_npdu_sampling with window w.  The
+def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float, w: int = -1,
+                 h: int = -1) -> dict:
+    """w < 0: fps_sampling; else fps_npdu_sampling with window w; h >= 0:
+    bucket_fps_kdline with tree height h instead.  The
     reference (oracle/_ref) single-threaded on one pinned core, over all m_full
     samples when the calibrated estimate fits target_s (else the largest
     prefix that does, stated in `sample`); the oracle's C restatement is timed
@@ -195,6 +208,10 @@ def cpu_baseline(pc: np.ndarray, m_full: int, target_s: float, w: int = -1) -> d
     ref = O.load_reference()
 
     def runner(which):
+        if h >= 0:
+            if which == "reference":
+                return lambda m: ref.bucket_fps_kdline(pc, m, h, 0)
+            return lambda m: O.kdline_quickfps(pc, m, h, 0)
         if which == "reference":
             if w < 0:
                 return lambda m: ref.fps_sampling(pc, m, 0)
@@ -763,6 +780,7 @@ def main() -> int:
     pc = wl["gen"]()
     n = pc.shape[0]
     npdu = wl.get("kind") == "npdu"
+    kdline = wl.get("kind") == "kdline"
     w = int(n / m * 16) if npdu else -1
     d_start = D.DeviceArray.from_numpy(np.zeros(1, np.uint64))
     d_out = D.DeviceArray((m,), np.uint64)
@@ -772,6 +790,15 @@ def main() -> int:
 
         def step():
             D.fps_npdu_sampling_device(d_pts, 3, 1, n, 3, m, w, d_start, d_out)
+    elif kdline:
+        import fpsample_amd as F
+
+        plan = dict(D.plan_info(n, 3, tie=D.TIE_LOWEST, n_samples=m), tie="lowest",
+                    h=wl["h"])
+        host_out = [None]
+
+        def step():
+            host_out[0] = F.bucket_fps_kdline_sampling(pc, m, wl["h"], start_idx=0)
     else:
         soa = np.ascontiguousarray(pc.T)  # (3, N): coordinate j of point i at j*N + i
         d_pts = D.DeviceArray.from_numpy(soa)
@@ -798,8 +825,9 @@ def main() -> int:
     D.timing(False)
     D.device_status()
 
-    out = d_out.numpy()
-    assert out[0] == 0 and (npdu or len(np.unique(out)) == m), "picks failed the sanity check"
+    out = host_out[0] if kdline else d_out.numpy()
+    assert (kdline or out[0] == 0) and (npdu or len(np.unique(out)) == m), \
+        "picks failed the sanity check"
 
     wall = t1 - t0
     ms_per_step = wall / args.steps * 1e3
@@ -840,11 +868,14 @@ def main() -> int:
             "value": round(m / mean_s, 3), "unit": "samples/s", "cores": 1,
             "hardware": "Xeon Gold 6326 @ 2.90 GHz", "source": where,
             "gpu_over_published": round(value / (m / mean_s), 2)}
+    if kdline:
+        result["note"] = ("host arrays in and out: each step is the whole call, the host "
+                          "KD-line build, the copies over PCIe and the GPU launch")
     if args.cpu_baseline == "on":
-        result["cpu_baseline"] = cpu_baseline(pc, m, args.cpu_seconds, w)
+        result["cpu_baseline"] = cpu_baseline(pc, m, args.cpu_seconds, w, wl.get("h", -1))
         if args.workload == "gauss1m":
             result.update(accuracy_and_bucket(pc, out, m))
-        elif not npdu and n <= 65536:
+        elif not npdu and not kdline and n <= 65536:
             from oracle import oracle as O
 
             result["accuracy_vs_fp64"] = O.accuracy_vs_f64(pc, out, 0)

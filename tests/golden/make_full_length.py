@@ -14,6 +14,9 @@ the cloud itself:
   uniform50k  N = 50000 uniform in [0,1)^3 (np.random.seed(42)), M = 4096, start 0:
             the reference's own 50k bench case (bench/test_bench.py:89-93)
   uniform100k N = 100000, same generator, M = 50000, start 0 (bench/test_bench.py:143-147)
+  kdline50k, kdline100k  the same two clouds through the reference's bucket_fps_kdline
+            (src/wrapper.hpp:118-132) with h = 7, start position 0
+            (bench/test_bench.py:131-135, 178-182)
   batch256  the 256 ragged clouds (default_rng(2)), M_i = N_i // 4: per cloud the
             SHA-256 of its picks (uint64 little-endian) and its first 16 picks
   serve96   the 96 serving-stream requests (default_rng(3)), M = N * U{1/8,1/4,1/2}:
@@ -63,12 +66,15 @@ def _single(task):
 
     if name.startswith("lattice"):
         pc = bench.gen_lattice_dups(n)
-    elif name.startswith("uniform"):
+    elif name.startswith("uniform") or name in KDLINE:
         pc = bench.gen_uniform_n(n)
     else:
         pc = bench.gen_gauss_mixture(n)
     t0 = time.time()
-    out = np.asarray(_ref().fps_sampling(pc, m, 0), np.uint64)
+    if name in KDLINE:
+        out = np.asarray(_ref().bucket_fps_kdline(pc, m, KDLINE[name], 0), np.uint64)
+    else:
+        out = np.asarray(_ref().fps_sampling(pc, m, 0), np.uint64)
     assert out.max() < 2**32
     np.savez_compressed(os.path.join(OUT, name + ".npz"), n=np.int64(n), m=np.int64(m),
                         start=np.uint64(0), cloud_sha256=np.array(sha(pc)),
@@ -99,7 +105,10 @@ def _many(name, clouds, ms, pool):
 
 SINGLES = {"lattice16m": (1 << 24, 4096), "gauss2m": (1 << 21, 16384),
            "gauss1m": (1 << 20, 16384), "gauss1m1": ((1 << 20) + 1, 16384),
-           "uniform100k": (100_000, 50_000), "uniform50k": (50_000, 4096)}
+           "uniform100k": (100_000, 50_000), "uniform50k": (50_000, 4096),
+           "kdline100k": (100_000, 50_000), "kdline50k": (50_000, 4096)}
+# bucket_fps_kdline (src/wrapper.hpp:118-132) with these tree heights
+KDLINE = {"kdline100k": 7, "kdline50k": 7}
 
 
 def main(names):

@@ -112,6 +112,19 @@ def test_full_length_uniform50k():
     np.testing.assert_array_equal(O.fps(pc, int(d["m"]), 0), d["out"])
 
 
+def test_full_length_kdline50k():
+    # Every pick of the reference's bucket_fps_kdline (src/wrapper.hpp:118-132) on
+    # its own 50k h=7 bench case (bench/test_bench.py:131-135), start position 0.
+    import bench
+    from tests.golden import full_length as FL
+
+    d = FL.load("kdline50k")
+    pc = bench.gen_uniform_n(50_000)
+    assert FL.sha256(pc) == d["cloud_sha256"]
+    np.testing.assert_array_equal(O.kdline_quickfps(pc, int(d["m"]), 7, 0), d["out"])
+    np.testing.assert_array_equal(O.kdline_exhaustive(pc, int(d["m"]), 7, 0), d["out"])
+
+
 def test_numpy_and_c_agree_on_ties():
     rng = np.random.default_rng(7)
     pc = rng.integers(0, 4, (400, 3)).astype(np.float32)
