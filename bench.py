@@ -144,9 +144,14 @@ WORKLOADS = {
     "lattice16m": dict(gen=gen_lattice_dups, m=4096,
                        desc="one cloud N=16777216 D=3 f32 lattice {0..255}^3, 25% duplicates, "
                             "M=4096, start_idx=0"),
-    # bucket_fps_kdline_sampling (h = 7) on the same two clouds (bench/test_bench.py:131-135,
-    # 178-182): host arrays in and out, so each step includes the host KD-line build and
+    # bucket_fps_kdline_sampling on the same clouds (h = 5 at 4k, 7 at 50k and 100k, the
+    # reference's fastest heights there; bench/test_bench.py:70-74, 131-135, 178-182): host
+    # arrays in and out, so each step includes the host KD-line build and
     # the PCIe copies; the README publishes the reference's mean times
+    "kdline4k": dict(gen=gen_uniform_4096, m=1024, kind="kdline", h=5,
+                     desc="bucket_fps_kdline_sampling h=5 on host arrays, one cloud N=4096 "
+                          "D=3 f32 uniform [0,1)^3 seed 42, M=1024, start_idx=0",
+                     published=(1.8301e-3, "README.md:117")),
     "kdline50k": dict(gen=lambda: gen_uniform_n(50_000), m=4096, kind="kdline", h=7,
                       desc="bucket_fps_kdline_sampling h=7 on host arrays, one cloud N=50000 "
                            "D=3 f32 uniform [0,1)^3 seed 42, M=4096, start_idx=0",

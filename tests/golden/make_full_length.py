@@ -16,7 +16,8 @@ the cloud itself:
   uniform100k N = 100000, same generator, M = 50000, start 0 (bench/test_bench.py:143-147)
   kdline50k, kdline100k  the same two clouds through the reference's bucket_fps_kdline
             (src/wrapper.hpp:118-132) with h = 7, start position 0
-            (bench/test_bench.py:131-135, 178-182)
+            (bench/test_bench.py:131-135, 178-182); kdline4k the 4k cloud with h = 5
+            (bench/test_bench.py:70-74)
   batch256  the 256 ragged clouds (default_rng(2)), M_i = N_i // 4: per cloud the
             SHA-256 of its picks (uint64 little-endian) and its first 16 picks
   serve96   the 96 serving-stream requests (default_rng(3)), M = N * U{1/8,1/4,1/2}:
@@ -106,9 +107,10 @@ def _many(name, clouds, ms, pool):
 SINGLES = {"lattice16m": (1 << 24, 4096), "gauss2m": (1 << 21, 16384),
            "gauss1m": (1 << 20, 16384), "gauss1m1": ((1 << 20) + 1, 16384),
            "uniform100k": (100_000, 50_000), "uniform50k": (50_000, 4096),
-           "kdline100k": (100_000, 50_000), "kdline50k": (50_000, 4096)}
+           "kdline100k": (100_000, 50_000), "kdline50k": (50_000, 4096),
+           "kdline4k": (4096, 1024)}
 # bucket_fps_kdline (src/wrapper.hpp:118-132) with these tree heights
-KDLINE = {"kdline100k": 7, "kdline50k": 7}
+KDLINE = {"kdline100k": 7, "kdline50k": 7, "kdline4k": 5}
 
 
 def main(names):

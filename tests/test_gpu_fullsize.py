@@ -9,7 +9,7 @@ hold every pick of
   - the 16.7 M-point lattice with 25 % duplicates, 4,096 picks (heavy ties);
   - the reference's own 50k and 100k bench cases (uniform, np.random.seed(42);
     4,096 of 50,000 and 50,000 of 100,000 picks), through fps_sampling and
-    through bucket_fps_kdline_sampling with h = 7;
+    through bucket_fps_kdline_sampling with h = 7, and the 4k case with h = 5;
 and a SHA-256 per cloud of every pick of
   - the 256 ragged clouds of the batch workload (M_i = N_i / 4);
   - the 96 requests of the serving workload (N up to 2^20, M = N/8..N/2).
@@ -101,9 +101,10 @@ def test_reference_bench_cases_every_pick(gpu, monkeypatch, n, name, env):
     check_single(gpu, monkeypatch, name, bench.gen_uniform_n(n), env)
 
 
-@pytest.mark.parametrize("n,name", [(50_000, "kdline50k"), (100_000, "kdline100k")])
-def test_reference_kdline_bench_cases_every_pick(gpu, monkeypatch, n, name):
-    # bucket_fps_kdline_sampling with h = 7 on the same two clouds: the KD-line
+@pytest.mark.parametrize("n,name,h", [(4096, "kdline4k", 5), (50_000, "kdline50k", 7),
+                                      (100_000, "kdline100k", 7)])
+def test_reference_kdline_bench_cases_every_pick(gpu, monkeypatch, n, name, h):
+    # bucket_fps_kdline_sampling with the reference's bench heights: the KD-line
     # order built on the host, lowest-position ties and FLT_MAX initial
     # distances on the GPU, against the reference's bucket search.
     import fpsample_amd as F
@@ -114,8 +115,8 @@ def test_reference_kdline_bench_cases_every_pick(gpu, monkeypatch, n, name):
     for k in ("FPSX_PLAN", "FPSX_NO_PAIR", "FPSX_NO_PRUNE"):
         monkeypatch.delenv(k, raising=False)
     m = int(d["m"])
-    log(f"{name}: n={n} m={m} h=7 plan={gpu.plan_info(n, 3, m=m, tie=1)}")
-    out = F.bucket_fps_kdline_sampling(pc, m, 7, start_idx=0)
+    log(f"{name}: n={n} m={m} h={h} plan={gpu.plan_info(n, 3, m=m, tie=1)}")
+    out = F.bucket_fps_kdline_sampling(pc, m, h, start_idx=0)
     i = first_diff(out, d["out"])
     assert i < 0, f"{name}: first differing pick {i} (got {out[i]}, reference {d['out'][i]})"
 
