@@ -160,14 +160,20 @@ WORKLOADS = {
                        desc="bucket_fps_kdline_sampling h=7 on host arrays, one cloud N=100000 "
                             "D=3 f32 uniform [0,1)^3 seed 42, M=50000, start_idx=0",
                        published=(233.3726e-3, "README.md:142")),
-    # windowed-update FPS on the reference's own bench cases (bench/test_bench.py:43-46,
-    # 151-154): w is the Python wrapper's default int(N / M * 16)
+    # windowed-update FPS on the reference's own bench cases (bench/test_bench.py:42-46,
+    # 96-100, 150-154): w is the Python wrapper's default int(N / M * 16)
     "npdu4k": dict(gen=gen_uniform_4096, m=1024, kind="npdu",
                    desc="fps_npdu_sampling, one cloud N=4096 D=3 f32 uniform [0,1)^3 seed 42, "
-                        "M=1024, w=64, start_idx=0"),
+                        "M=1024, w=64, start_idx=0",
+                   published=(3.9514e-3, "README.md:120")),
+    "npdu50k": dict(gen=lambda: gen_uniform_n(50_000), m=4096, kind="npdu",
+                    desc="fps_npdu_sampling, one cloud N=50000 D=3 f32 uniform [0,1)^3 seed 42, "
+                         "M=4096, w=195, start_idx=0",
+                    published=(180.5911e-3, "README.md:134")),
     "npdu100k": dict(gen=lambda: gen_uniform_n(100_000), m=50_000, kind="npdu",
                      desc="fps_npdu_sampling, one cloud N=100000 D=3 f32 uniform [0,1)^3 seed 42, "
-                          "M=50000, w=32, start_idx=0"),
+                          "M=50000, w=32, start_idx=0",
+                     published=(4333.3156e-3, "README.md:147")),
 }
 
 
@@ -866,15 +872,14 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": None,
     }
-    if not npdu:
-        try:
-            from tests.golden import full_length as FL
+    try:
+        from tests.golden import full_length as FL
 
-            if FL.available(args.workload):
-                result["identical_to_reference"] = bool(
-                    np.array_equal(out, FL.load(args.workload)["out"]))
-        except ImportError:
-            pass
+        if FL.available(args.workload):
+            result["identical_to_reference"] = bool(
+                np.array_equal(out, FL.load(args.workload)["out"]))
+    except ImportError:
+        pass
     if "published" in wl:
         mean_s, where = wl["published"]
         result["reference_published"] = {

@@ -11,13 +11,18 @@ the cloud itself:
   gauss1m1  the same generator at N = 2^20 + 1
   gauss2m   the same generator at N = 2^21
   lattice16m  N = 2^24 lattice {0..255}^3 with 25 % duplicates (default_rng(4)), M = 4096
-  uniform50k  N = 50000 uniform in [0,1)^3 (np.random.seed(42)), M = 4096, start 0:
+  uniform4k   N = 4096 uniform in [0,1)^3 (np.random.seed(42)), M = 1024, start 0
+            (bench/test_bench.py:29-33)
+  uniform50k  N = 50000, same generator, M = 4096, start 0:
             the reference's own 50k bench case (bench/test_bench.py:89-93)
   uniform100k N = 100000, same generator, M = 50000, start 0 (bench/test_bench.py:143-147)
   kdline50k, kdline100k  the same two clouds through the reference's bucket_fps_kdline
             (src/wrapper.hpp:118-132) with h = 7, start position 0
             (bench/test_bench.py:131-135, 178-182); kdline4k the 4k cloud with h = 5
             (bench/test_bench.py:70-74)
+  npdu4k, npdu50k, npdu100k  the 4k, 50k and 100k clouds through the reference's
+            fps_npdu_sampling with the wrapper's default window, start 0
+            (bench/test_bench.py:42-46, 96-100, 150-154)
   batch256  the 256 ragged clouds (default_rng(2)), M_i = N_i // 4: per cloud the
             SHA-256 of its picks (uint64 little-endian) and its first 16 picks
   serve96   the 96 serving-stream requests (default_rng(3)), M = N * U{1/8,1/4,1/2}:
@@ -67,13 +72,15 @@ def _single(task):
 
     if name.startswith("lattice"):
         pc = bench.gen_lattice_dups(n)
-    elif name.startswith("uniform") or name in KDLINE:
+    elif name.startswith("uniform") or name in KDLINE or name in NPDU:
         pc = bench.gen_uniform_n(n)
     else:
         pc = bench.gen_gauss_mixture(n)
     t0 = time.time()
     if name in KDLINE:
         out = np.asarray(_ref().bucket_fps_kdline(pc, m, KDLINE[name], 0), np.uint64)
+    elif name in NPDU:
+        out = np.asarray(_ref().fps_npdu(pc, m, NPDU[name], 0), np.uint64)
     else:
         out = np.asarray(_ref().fps_sampling(pc, m, 0), np.uint64)
     assert out.max() < 2**32
@@ -107,10 +114,15 @@ def _many(name, clouds, ms, pool):
 SINGLES = {"lattice16m": (1 << 24, 4096), "gauss2m": (1 << 21, 16384),
            "gauss1m": (1 << 20, 16384), "gauss1m1": ((1 << 20) + 1, 16384),
            "uniform100k": (100_000, 50_000), "uniform50k": (50_000, 4096),
+           "uniform4k": (4096, 1024),
            "kdline100k": (100_000, 50_000), "kdline50k": (50_000, 4096),
-           "kdline4k": (4096, 1024)}
+           "kdline4k": (4096, 1024),
+           "npdu4k": (4096, 1024), "npdu50k": (50_000, 4096), "npdu100k": (100_000, 50_000)}
 # bucket_fps_kdline (src/wrapper.hpp:118-132) with these tree heights
 KDLINE = {"kdline100k": 7, "kdline50k": 7, "kdline4k": 5}
+# fps_npdu_sampling (src/lib.cpp:272-340) with the Python wrapper's default window
+# int(N / M * 16) (src/fpsample/__init__.py:97)
+NPDU = {"npdu4k": 64, "npdu50k": 195, "npdu100k": 32}
 
 
 def main(names):

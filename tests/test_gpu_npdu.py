@@ -166,6 +166,28 @@ def test_npdu_pair_reference_benchmark_size(gpu, npdu_plan):
     np.testing.assert_array_equal(two[:2000], O.npdu(pc, 2000, 32, 0))
 
 
+@pytest.mark.parametrize("n,m,w,name", [(4096, 1024, 64, "npdu4k"), (50_000, 4096, 195, "npdu50k"),
+                                        (100_000, 50_000, 32, "npdu100k")],
+                         ids=["4k", "50k", "100k"])
+@pytest.mark.parametrize("plan", [None, "3:1024:8", "stream"], ids=["auto", "one_pick", "stream"])
+def test_npdu_reference_bench_cases_every_pick(gpu, npdu_plan, n, m, w, name, plan):
+    # The reference's own NPDU bench cases (bench/test_bench.py:42-46, 96-100,
+    # 150-154; the wrapper's default window int(N / M * 16)), every pick against
+    # the reference's output (full-length fixtures), under the planner's choice,
+    # the one-pick register kernel and the streaming form.
+    import bench
+    from tests.golden import full_length as FL
+
+    d = FL.load(name)
+    pc = bench.gen_uniform_n(n)
+    assert FL.sha256(pc) == d["cloud_sha256"]
+    npdu_plan(plan)
+    log(f"{name}: n={n} m={m} w={w} plan={plan} info={gpu.npdu_plan_info(n, 3)}")
+    out = run(gpu, pc, m, w, 0)
+    bad = np.nonzero(out != d["out"])[0]
+    assert bad.size == 0, f"{name} {plan}: first differing pick {bad[0]}"
+
+
 def test_npdu_python_api_and_device_entry(gpu):
     import fpsample_amd as F
     from fpsample_amd import device as D
