@@ -9,7 +9,12 @@ sorts, gather), and the single persistent FPS launch that makes all M-1
 passes.  The kernel shape comes from the library's planner; --autotune on
 times every plan first (untimed) instead.  The npdu* workloads time
 fps_npdu_sampling (src/lib.cpp:272-340) the same way, through
-fpsx_fps_npdu_sampling_device on row-major device data.
+fpsx_fps_npdu_sampling_device on row-major device data.  The kdline*
+workloads time bucket_fps_kdline_sampling (src/fpsample/__init__.py:174-206),
+whose entry takes host arrays: each step there is the whole call, the host
+KD-line build and the PCIe copies included.  Workloads taken from the
+reference's own bench (bench/test_bench.py) also report the mean time its
+README publishes for them (reference_published).
 
 Default workload (the metric's configuration): one cloud of N = 1,048,576
 points, D = 3 float32, a mixture of 64 isotropic Gaussian clusters (sigma
