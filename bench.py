@@ -433,14 +433,15 @@ def roofline(workload: str, plan: dict, n: int, m: int, kernel_s: float, npdu: b
         out.update(bound="latency", achieved=None, peak=None, unit="exchanges/s", frac=None,
                    note="one workgroup, no cross-workgroup exchange")
         return out
+    exchanges = launch_exchanges()
     if npdu:
-        exchanges = None
-        words = 5
+        # The paired windowed-update kernel (10-word candidates) settles up to
+        # two picks per exchange; the one-pick kernels (5 words) make m - 1.
+        words = 10 if exchanges and exchanges < m - 1 else 5
     else:
-        exchanges = launch_exchanges()
         words = 10 if kind in ("pair", "spair") else 5
     floor_us = probe_exchange(grid, words)
-    if exchanges is None:
+    if not exchanges:
         out.update(bound="latency", achieved=None, peak=round(1e6 / floor_us, 1),
                    unit="exchanges/s", frac=None, exchange_floor_us=round(floor_us, 4))
         return out

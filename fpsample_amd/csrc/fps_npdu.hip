@@ -601,6 +601,8 @@ __global__ void __launch_bounds__(B) fps_npdu_pair(NpduParams a) {
     }
     s += have2 ? 2u : 1u;
   }
+  // The launch's exchange count (one per round), read by fpsx_launch_exchanges.
+  if (recorder && lane == 0) f.status[1] = e;
 }
 
 // ---------------------------------------------------------------------------

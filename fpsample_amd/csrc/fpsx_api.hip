@@ -748,6 +748,9 @@ int launch_npdu(Ctx &c, const Plan &p, const float *d_pts, size_t sp, size_t ld,
   }
   rc = dispatch(c, p, &a, st);
   if (rc) return rc;
+  c.last_plan = p;
+  c.last_m = m;
+  c.last_starts = 1;
   return scratch_release(c, st);
 }
 
@@ -987,7 +990,12 @@ FPSX_EXPORT int fpsx_launch_exchanges(void *stream, uint64_t *exchanges) {
   *exchanges = 0;
   if (!c.last_plan.k) return FPSX_OK;
   if (c.last_plan.grid < 2 && c.last_plan.kind != PlanKind::Pair) return FPSX_OK;
-  if (c.last_plan.kind != PlanKind::Pair && c.last_plan.kind != PlanKind::StreamPair) {
+  size_t npdu_pairs = 0;
+  const KernelEntry *npdu_pair = npdu_pair_table(&npdu_pairs);
+  const bool paired_npdu = c.last_plan.kind == PlanKind::Npdu && c.last_plan.k >= npdu_pair &&
+                           c.last_plan.k < npdu_pair + npdu_pairs;
+  if (c.last_plan.kind != PlanKind::Pair && c.last_plan.kind != PlanKind::StreamPair &&
+      !paired_npdu) {
     // One exchange per pick after the forced prefix.
     *exchanges = c.last_m > c.last_starts ? c.last_m - c.last_starts : 0;
     return FPSX_OK;

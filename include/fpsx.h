@@ -119,13 +119,13 @@ int fpsx_autotune(const float *d_points, size_t stride_point, size_t stride_dim,
 int fpsx_exchange_timeout(double seconds);
 
 /*
- * Number of cross-workgroup exchanges the last FPS launch on the current
- * device made (after `stream`, NULL: the library's, reaches it): 0 for a
- * one-workgroup plan (but for kind 5 with one workgroup: its rounds, one
- * barrier each), one per pick after the forced prefix for the one-pick
- * kernels, counted by the kernel for the paired kinds (up to two picks per
- * exchange; kind 5 also exchanges workgroup boxes once).  Measurement aid; no
- * counterpart in the reference.
+ * Number of cross-workgroup exchanges the last FPS or windowed-update launch
+ * on the current device made (after `stream`, NULL: the library's, reaches
+ * it): 0 for a one-workgroup plan (but for kind 5 with one workgroup: its
+ * rounds, one barrier each), one per pick after the forced prefix for the
+ * one-pick kernels, counted by the kernel for the paired kinds and the paired
+ * windowed-update kernel (up to two picks per exchange; kind 5 also exchanges
+ * workgroup boxes once).  Measurement aid; no counterpart in the reference.
  */
 int fpsx_launch_exchanges(void *stream, uint64_t *exchanges);
 

@@ -188,6 +188,29 @@ def test_npdu_reference_bench_cases_every_pick(gpu, npdu_plan, n, m, w, name, pl
     assert bad.size == 0, f"{name} {plan}: first differing pick {bad[0]}"
 
 
+def test_npdu_exchange_count(gpu, npdu_plan):
+    # fpsx_launch_exchanges after a windowed-update launch: the paired kernel
+    # counts its rounds (between (m - 1) / 2 and m - 1 exchanges), the one-pick
+    # kernel makes one per pick after the start.
+    import ctypes
+
+    import bench
+
+    L = gpu.L
+    L.fpsx_launch_exchanges.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+    x = ctypes.c_uint64()
+    pc = bench.gen_uniform_n(50_000)
+    for plan, pair in ((None, True), ("3:1024:8", False)):
+        npdu_plan(plan)
+        run(gpu, pc, 4096, 195, 0)
+        assert L.fpsx_launch_exchanges(None, ctypes.byref(x)) == 0, gpu.last_error()
+        log(f"npdu 50k plan={plan}: {x.value} exchanges")
+        if pair:
+            assert (4096 - 1) // 2 <= x.value < 4096 - 1
+        else:
+            assert x.value == 4096 - 1
+
+
 def test_npdu_python_api_and_device_entry(gpu):
     import fpsample_amd as F
     from fpsample_amd import device as D
