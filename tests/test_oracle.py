@@ -315,3 +315,25 @@ def test_coverage_radius_definition_and_consistency():
         assert abs(O.coverage_radius(pc, ref[:k]) - np.sqrt(dist[k])) <= 1e-12
     acc = O.accuracy_vs_f64(pc, O.fps(pc, 59, 4), 4)
     assert acc["index_agreement"] == 1.0 and acc["coverage_radius_rel_err"] == 0.0
+
+
+def test_reference_bench_workloads_have_full_length_fixtures():
+    # Every bench workload taken from the reference's own bench (those that
+    # carry its published time) has a full-length fixture of the reference's
+    # picks, made from the same generator, so its bench line reports
+    # identical_to_reference; the window of an NPDU case is the wrapper's
+    # default int(N / M * 16) (src/fpsample/__init__.py:97).
+    import bench
+    from tests.golden import full_length as FL
+
+    names = [k for k, wl in bench.WORKLOADS.items() if "published" in wl]
+    assert len(names) == 9
+    for name in names:
+        wl = bench.WORKLOADS[name]
+        d = FL.load(name)
+        pc = wl["gen"]()
+        assert FL.sha256(pc) == d["cloud_sha256"], name
+        assert int(d["m"]) == wl["m"] and int(d["n"]) == pc.shape[0], name
+        assert len(d["out"]) == wl["m"], name
+        if wl.get("kind") == "npdu":
+            assert f"w={int(pc.shape[0] / wl['m'] * 16)}," in wl["desc"], name
